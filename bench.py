@@ -1,0 +1,254 @@
+"""Benchmark: PTE closure cells/s on one MI355X.
+
+Headline (`value`): batched uneven closure — 2^20 mixed cells with 2..6
+materials from a pool of six (IdealGas x2, StiffGas, Gruneisen, Vinet, a
+Gruneisen-sampled rho-T table), PTESolverRhoT at the 1e-10 tolerances of
+singularity-eos/example/pte_2mat.cpp:169-180, FP64, one hipGraph launch per
+step (persistent kernel per material-count bin). A step restores the solver
+inputs (device-to-device copies; the closure works in place) and solves every
+cell; inputs are resident in HBM before timing starts.
+
+Reported beside it: the plain per-call path on the same cells and on the
+65,536-cell two-material closure, the 2^24-query table lookups, the
+roofline of the dominant bin kernel (HIP events on the launch stream) and the
+oracle CPU baseline on a bounded sample of the same workload.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "PTE closure cells/s at 1e-10 tol + p99 request latency, served vs plain"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_cell(nm):
+    """Algorithmic HBM bytes of one closure: reads rho, vfrac, temp, press per
+    material, sie_tot, vfrac_tot, the CSR offset and material ids; writes rho,
+    vfrac, sie, temp, press per material and the converged flag."""
+    return 8 * 4 * nm + 16 + 4 + nm + 8 * 5 * nm + 4
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ncell", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the closure shards by cell "
+                                   "ranges with no exchange (see DESIGN.md), --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    sgm = importlib.import_module("singularity-eos_amd")
+    wl = importlib.import_module("singularity-eos_amd.workloads")
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    params = sgm.mix_params(tight=True)
+
+    ctx, table = wl.pool_context()
+    log(f"[bench] building uneven workload: {args.ncell} cells")
+    a = wl.uneven(ctx, args.ncell)
+    n, nnz = a["ncell"], a["nnz"]
+    nm_cell = np.diff(a["offsets"])
+    fields = ("rho", "vfrac", "sie", "temp", "press")
+    init = {k: torch.tensor(a[k], device=dev) for k in fields}
+    work = {k: v.clone() for k, v in init.items()}
+    vt = torch.tensor(a["vfrac_tot"], device=dev)
+    st = torch.tensor(a["sie_tot"], device=dev)
+    off = torch.tensor(a["offsets"], device=dev)
+    mat = torch.tensor(a["mat_ids"], device=dev)
+    conv = torch.zeros(n, dtype=torch.int32, device=dev)
+    niter = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    plan = sgm.Plan(ctx, n, nnz, off, mat, 0, stream=stream)
+    if args.autotune:
+        log("[bench] autotuning launch shapes")
+        plan.autotune(init["rho"], init["vfrac"], init["temp"], init["press"], vt, st,
+                      params=params, reps=1, stream=stream)
+    bins = plan.describe()
+
+    def restore():
+        for k in fields:
+            work[k].copy_(init[k], non_blocking=True)
+
+    def step():
+        restore()
+        rc = plan.execute(work["rho"], work["vfrac"], work["sie"], work["temp"], work["press"], vt,
+                          st, params=params, conv=conv, niter=niter, stream=stream)
+        assert rc == 0, rc
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    cells_per_s = n / dt
+    conv_frac = float(conv.float().mean().item())
+    mean_iter = float(niter.float().mean().item())
+    log(f"[bench] batched: {dt * 1e3:.3f} ms/step, {cells_per_s:.4g} cells/s, "
+        f"converged {conv_frac:.5f}, mean iters {mean_iter:.2f}")
+
+    # per-bin kernel timing (HIP events on the launch stream)
+    per_bin = []
+    for bi, b in enumerate(bins):
+        restore()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = 3
+        ms = 0.0
+        for _ in range(reps):
+            restore()
+            e0.record()
+            rc = plan.execute_bin(bi, work["rho"], work["vfrac"], work["sie"], work["temp"],
+                                  work["press"], vt, st, params=params, conv=conv, niter=niter,
+                                  stream=stream)
+            e1.record()
+            assert rc == 0
+            torch.cuda.synchronize()
+            ms += e0.elapsed_time(e1)
+        ms /= reps
+        sel = np.zeros(n, dtype=bool)
+        cells = b["ncells"]
+        nm = b["nmat"]
+        per_bin.append(dict(nmat=nm, cells=cells, ms=ms, block=b["block"],
+                            cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
+    dom = max(per_bin, key=lambda r: r["ms"])
+    nm = dom["nmat"] if dom["nmat"] else 4
+    alg_bytes = dom["cells"] * bytes_per_cell(nm)
+    achieved = alg_bytes / (dom["ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": f"pte_bin_kernel<{dom['nmat']},RhoT>", "cells": dom["cells"],
+                "ms": dom["ms"], "bytes_per_cell": bytes_per_cell(nm)}
+
+    extras = {"bins": per_bin, "converged_frac": conv_frac, "mean_newton_iters": mean_iter}
+    if not args.no_extras:
+        # plain per-call path, same cells
+        restore()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            restore()
+            rc = ctx.pte(0, n, nnz, off, mat, work["rho"], work["vfrac"], work["sie"],
+                         work["temp"], work["press"], vt, st, params=params, conv=conv,
+                         stream=stream)
+            assert rc == 0
+        torch.cuda.synchronize()
+        extras["plain_uneven_cells_per_s"] = n * reps / (time.perf_counter() - t0)
+        # two-material closure, plain per-call path (65,536 cells)
+        c0 = wl.pte2(65536)
+        c0["mat_ids"] = np.tile(np.array([0, 2], dtype=np.uint8), 65536)
+        d0 = {k: torch.tensor(c0[k], device=dev) for k in fields + ("vfrac_tot", "sie_tot")}
+        i0 = {k: d0[k].clone() for k in fields}
+        o0 = torch.tensor(c0["offsets"], device=dev)
+        m0 = torch.tensor(c0["mat_ids"], device=dev)
+        for it in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                for k in fields:
+                    d0[k].copy_(i0[k])
+                ctx.pte(0, 65536, 131072, o0, m0, d0["rho"], d0["vfrac"], d0["sie"], d0["temp"],
+                        d0["press"], d0["vfrac_tot"], d0["sie_tot"], params=params, stream=stream)
+            torch.cuda.synchronize()
+            extras["plain_pte2_65536_cells_per_s"] = 65536 * 5 / (time.perf_counter() - t0)
+        # table lookups: 2^24 forward (P, e) and inverse T(rho, e)
+        nq = 1 << 24
+        rq, Tq = wl.table_queries(nq, table)
+        rq = torch.tensor(rq, device=dev)
+        Tq = torch.tensor(Tq, device=dev)
+        Pq, eq, T2 = torch.empty_like(rq), torch.empty_like(rq), torch.empty_like(rq)
+        for _ in range(2):
+            ctx.fill_eos(5, rq, Tq, eq, Pq, None, None, nq, 2 | 4, stream=stream)
+            ctx.vec_eval(5, 1, rq, eq, T2, stream=stream)
+        torch.cuda.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        ctx.fill_eos(5, rq, Tq, eq, Pq, None, None, nq, 2 | 4, stream=stream)
+        e1.record()
+        ctx.vec_eval(5, 1, rq, eq, T2, stream=stream)
+        e2.record()
+        torch.cuda.synchronize()
+        fwd_ms, inv_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+        extras["table_forward_queries_per_s"] = nq / (fwd_ms * 1e-3)
+        extras["table_forward_GBs"] = nq * 32 / (fwd_ms * 1e-3) / 1e9
+        extras["table_inverse_queries_per_s"] = nq / (inv_ms * 1e-3)
+        # analytic vector P(rho, e) on 2^24 points
+        Pa = torch.empty_like(rq)
+        ctx.vec_eval(2, 0, rq, eq, Pa, stream=stream)
+        torch.cuda.synchronize()
+        e0.record()
+        ctx.vec_eval(2, 0, rq, eq, Pa, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        vms = e0.elapsed_time(e1)
+        extras["vector_P_rho_sie_GBs"] = nq * 24 / (vms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        O.build()
+        t = table
+        otab, _, _ = O.gruneisen_table(wl.GRUN_AL, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin, t.lTMax,
+                                       t.nT)
+        olist = [O.ideal(*wl.IDEAL_A), O.ideal(*wl.IDEAL_M1), O.stiff(*wl.STIFF_B),
+                 O.gruneisen(*wl.GRUN_AL), O.vinet(*wl.VINET_CU), otab.eos()]
+        ns = 1 << 16
+        oo = a["offsets"][: ns + 1]
+        k = oo[-1]
+        r = {f: a[f][:k].copy() for f in fields}
+        threads = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        O.pte_solve(0, olist, oo, a["mat_ids"][:k].astype(np.int32), r["rho"], r["vfrac"],
+                    r["sie"], r["temp"], r["press"], a["vfrac_tot"][:ns], a["sie_tot"][:ns],
+                    params.as_oracle_array(), nthreads=threads)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": ns / cdt, "unit": "cells/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} cells of the same 2^20-cell uneven workload, "
+                         f"PTESolverRhoT at 1e-10 tol, OpenMP over cells ({cdt:.2f} s)"}
+        log(f"[bench] cpu baseline: {cpu['value']:.4g} cells/s on {threads} threads")
+
+    out = {
+        "metric": METRIC, "value": cells_per_s, "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven PTE closure (PTESolverRhoT), 2^20 cells, "
+                               "2-6 materials from a pool of 6, 1e-10 tolerance, hipGraph",
+                   "cells": n, "nnz": nnz, "mean_nmat": float(nm_cell.mean()),
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
+    }
+    print(json.dumps(out))
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+/*
+ * sgm — MI355X-native EOS evaluation and mixed-cell PTE closure, C ABI.
+ *
+ * Every array argument is a DEVICE pointer owned by the caller (HIP memory on
+ * the context's GPU) unless the comment says "host". Calls are asynchronous on
+ * the given hipStream_t (0 = the default stream); results are ready after the
+ * stream is synchronised. Functions return 0 on success and a negative
+ * SGM_E_* code on a rejected argument, before anything is launched. Each
+ * declaration names the singularity-eos interface it replaces.
+ */
+#ifndef SGM_H_
+#define SGM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *sgm_stream_t; /* == hipStream_t */
+typedef struct sgm_ctx sgm_ctx;
+typedef struct sgm_plan sgm_plan;
+typedef struct sgm_server sgm_server;
+
+enum {
+  SGM_OK = 0,
+  SGM_E_ARG = -1,      /* null/invalid argument or size */
+  SGM_E_MAT = -2,      /* material index out of range or not set */
+  SGM_E_HIP = -3,      /* HIP runtime error */
+  SGM_E_KIND = -4,     /* operation not defined for this EOS kind */
+  SGM_E_NOGPU = -5,    /* no usable GPU */
+  SGM_E_BUDGET = -6    /* request larger than the server's memory budget */
+};
+
+/* EOS kinds (the material models of the singularity::Variant on this path) */
+enum { SGM_IDEAL = 0, SGM_STIFF = 1, SGM_GRUNEISEN = 2, SGM_VINET = 3, SGM_TABLE_RHOT = 4 };
+
+/* thermalqs bit flags (singularity-eos/base/constants.hpp:22-35) */
+enum {
+  SGM_Q_DENSITY = 1 << 0,
+  SGM_Q_SIE = 1 << 1,
+  SGM_Q_PRESSURE = 1 << 2,
+  SGM_Q_TEMPERATURE = 1 << 3,
+  SGM_Q_CV = 1 << 4,
+  SGM_Q_BMOD = 1 << 5
+};
+
+/* vector function selectors for sgm_vec_eval */
+enum {
+  SGM_F_P_RHO_SIE = 0,   /* PressureFromDensityInternalEnergy */
+  SGM_F_T_RHO_SIE = 1,   /* TemperatureFromDensityInternalEnergy */
+  SGM_F_SIE_RHO_T = 2,   /* InternalEnergyFromDensityTemperature */
+  SGM_F_P_RHO_T = 3,     /* PressureFromDensityTemperature */
+  SGM_F_CV_RHO_SIE = 4,  /* SpecificHeatFromDensityInternalEnergy */
+  SGM_F_CV_RHO_T = 5,    /* SpecificHeatFromDensityTemperature */
+  SGM_F_BMOD_RHO_SIE = 6,/* BulkModulusFromDensityInternalEnergy */
+  SGM_F_BMOD_RHO_T = 7,  /* BulkModulusFromDensityTemperature */
+  SGM_F_MIN_SIE_RHO = 8  /* MinInternalEnergyFromDensity (in2 unused) */
+};
+
+/* MixParams (singularity-eos/closure/mixed_cell_models.hpp:43-78). Fill with
+ * sgm_mix_params_default() and override fields. */
+typedef struct sgm_mix_params {
+  double derivative_eps;
+  double pte_rel_tolerance_p, pte_rel_tolerance_t, pte_rel_tolerance_e, pte_rel_tolerance_v;
+  double pte_abs_tolerance_p, pte_abs_tolerance_t;
+  double pte_slow_convergence_thresh;
+  double pte_rel_tolerance_p_sufficient, pte_rel_tolerance_t_sufficient;
+  double pte_rel_tolerance_e_sufficient, pte_rel_tolerance_v_sufficient;
+  double pte_abs_tolerance_p_sufficient, pte_abs_tolerance_t_sufficient;
+  double line_search_alpha, line_search_fac, vfrac_safety_fac, temperature_limit;
+  double default_tguess, min_dtde, pte_small_step_thresh, pte_max_dpdv;
+  int pte_max_iter_per_mat, line_search_max_iter, pte_small_step_tries;
+  int pte_reduced_system_exclude_idx, iterate_t_guess, pad;
+} sgm_mix_params;
+
+/* Per-cell solver status (SolverStatus, mixed_cell_models.hpp:80-87, as SoA).
+ * Any pointer may be NULL. err holds SGM_ERRB_* bits. */
+typedef struct sgm_pte_status {
+  int *converged;
+  int *niter;
+  double *residual;
+  int *err;
+} sgm_pte_status;
+enum {
+  SGM_ERRB_GRUN_MAXRHO = 1, SGM_ERRB_GRUN_PT = 2, SGM_ERRB_FILLEOS = 4,
+  SGM_ERRB_LINSOLVE = 8, SGM_ERRB_SMALLSTEP = 16, SGM_ERRB_INPUT = 32
+};
+
+/* In-memory rho-T table (replaces the SP5/HDF5 file of
+ * SpinerEOSDependsRhoT(filename, matid), eos_spiner_rho_temp.hpp:373-420).
+ * HOST pointers. Grids are uniform in NQT log10 space; 2-D fields are
+ * [nRho][nT] with T fastest; cold-curve fields are [nRho]. */
+typedef struct sgm_table_desc {
+  double lRhoMin, lRhoMax;
+  int nRho;
+  double lTMin, lTMax;
+  int nT;
+  double rho_offset, T_offset;
+  const double *P, *sie, *bMod, *dPdRho, *dPdE, *dEdRho, *dEdT;
+  const double *PCold, *sieCold, *bModCold;
+  int reproducible;
+  int pmin_vapor_dome;
+} sgm_table_desc;
+
+/* ---- context and materials --------------------------------------------- */
+/* replaces init_sg_eos (singularity-eos/eos/singularity_eos.hpp:27) */
+int sgm_init(int nmat, sgm_ctx **ctx);
+/* replaces finalize_sg_eos (singularity_eos.hpp:131) */
+int sgm_finalize(sgm_ctx *ctx);
+/* replaces init_sg_IdealGas (singularity_eos.hpp:29) */
+int sgm_set_IdealGas(sgm_ctx *ctx, int matindex, double gm1, double Cv);
+/* replaces init_sg_StiffGas (singularity_eos.hpp:57) */
+int sgm_set_StiffGas(sgm_ctx *ctx, int matindex, double gm1, double Cv, double Pinf,
+                     double qq);
+/* replaces init_sg_Gruneisen (singularity_eos.hpp:36); rho_max <= 0 selects
+ * the automatic 0.99*ComputeRhoMax of the reference constructor */
+int sgm_set_Gruneisen(sgm_ctx *ctx, int matindex, double C0, double s1, double s2, double s3,
+                      double G0, double b, double rho0, double T0, double P0, double Cv,
+                      double rho_max);
+/* replaces Vinet(rho0,T0,B0,BP0,A0,Cv0,E0,S0,expconsts) (eos_vinet.hpp:44-51);
+ * d2to40: HOST array of 39 coefficients */
+int sgm_set_Vinet(sgm_ctx *ctx, int matindex, double rho0, double T0, double B0, double BP0,
+                  double A0, double Cv0, double E0, double S0, const double *d2to40);
+/* replaces init_sg_SpinerDependsRhoT (singularity_eos.hpp:76) */
+int sgm_set_TableRhoT(sgm_ctx *ctx, int matindex, const sgm_table_desc *desc);
+/* host-side query: kind of a material, or SGM_E_MAT */
+int sgm_get_kind(const sgm_ctx *ctx, int matindex);
+/* host-side query: derived table curve (0 PMax 1 sielTMax 2 dEdTMax 3 gm1Max
+ * 4 rho_at_pmin 5 lTColdCrit 6 bModCold 7 dPdECold 8 dEdTCold 9 bMod[2-D]
+ * 10 scalars {PMin, rhoMax, TMax}); out is HOST, returns element count */
+long sgm_table_query(const sgm_ctx *ctx, int matindex, int which, double *out);
+
+/* ---- vector EOS calls --------------------------------------------------- */
+/* replaces get_sg_PressureFromDensityInternalEnergy (singularity_eos.hpp:103) and
+ * Variant::PressureFromDensityInternalEnergy(rhos, sies, pressures, num, lambdas)
+ * (eos_variant.hpp:543-545). lambda: NULL or [n*lambda_stride] with
+ * lambda[i*stride+0] = log10 rho, [+1] = log10 T (table EOS cache). */
+int sgm_vec_P_from_rho_sie(sgm_ctx *ctx, int matindex, const double *rho, const double *sie,
+                           double *P, long n, int lambda_stride, double *lambda,
+                           sgm_stream_t stream);
+/* replaces Variant::TemperatureFromDensityInternalEnergy (eos_variant.hpp:543) */
+int sgm_vec_T_from_rho_sie(sgm_ctx *ctx, int matindex, const double *rho, const double *sie,
+                           double *T, long n, int lambda_stride, double *lambda,
+                           sgm_stream_t stream);
+/* the remaining SG_EOS_VEC_2IN_1OUT vector calls (eos_base.hpp:169-230), by selector */
+int sgm_vec_eval(sgm_ctx *ctx, int matindex, int func, const double *in1, const double *in2,
+                 double *out, long n, int lambda_stride, double *lambda, sgm_stream_t stream);
+/* replaces Variant::FillEos(rhos, temps, energies, presses, cvs, bmods, num, output,
+ * lambdas) (eos_variant.hpp:652-708). Arrays not read or written for `output`
+ * may be NULL. errs: NULL or [n] SGM_ERRB_* bits. */
+int sgm_vec_fill_eos(sgm_ctx *ctx, int matindex, double *rho, double *temp, double *sie,
+                     double *press, double *cv, double *bmod, long n, unsigned long output,
+                     int lambda_stride, double *lambda, int *errs, sgm_stream_t stream);
+/* DensityEnergyFromPressureTemperature over arrays (eos_base.hpp:630-671 and
+ * overrides); rho is the in/out initial guess */
+int sgm_vec_rhoe_from_PT(sgm_ctx *ctx, int matindex, const double *P, const double *T,
+                         double *rho, double *sie, long n, int lambda_stride, double *lambda,
+                         int *errs, sgm_stream_t stream);
+
+/* ---- PTE closure (plain per-call path) ----------------------------------- */
+/* One launch over a CSR list of mixed cells: cell c owns entries
+ * [offsets[c], offsets[c+1]) of the per-(cell,material) arrays; mat_ids holds
+ * context material indices; 1..8 materials per cell. Replaces constructing
+ * PTESolverRhoT / PTESolverPT(nmat, eos, vfrac_tot, sie_tot, rho, vfrac, sie,
+ * temp, press, lambda, scratch, Tnorm, params) and calling PTESolver
+ * (mixed_cell_models.hpp:922-935, 1245-1253, 2188-2291) once per cell.
+ * Inputs: rho, vfrac (guess), temp and press (guesses; press used by
+ * pressure-preferring EOS), vfrac_tot[c], sie_tot[c], Tnorm[c] (NULL = 0).
+ * Outputs overwrite rho, vfrac, sie, temp, press. lambda: NULL or [nnz*3].
+ * params: HOST pointer (NULL = defaults). */
+int sgm_pte_rhoT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+                 double *rho, double *vfrac, double *sie, double *temp, double *press,
+                 const double *vfrac_tot, const double *sie_tot, const double *Tnorm,
+                 double *lambda, const sgm_mix_params *params, sgm_pte_status status,
+                 sgm_stream_t stream);
+int sgm_pte_PT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+               double *rho, double *vfrac, double *sie, double *temp, double *press,
+               const double *vfrac_tot, const double *sie_tot, const double *Tnorm,
+               double *lambda, const sgm_mix_params *params, sgm_pte_status status,
+               sgm_stream_t stream);
+
+/* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
+/* No reference counterpart: a plan bins the cells of a fixed CSR topology by
+ * material count (2..6 run register-resident templated kernels) and by
+ * material signature, autotunes block shape and cells-per-wave per bin, and
+ * captures one hipGraph whose persistent kernel nodes solve every bin.
+ * method: 0 = RhoT, 1 = PT. Results are identical to sgm_pte_rhoT / sgm_pte_PT
+ * on the same inputs. The plan keeps device copies of offsets and mat_ids. */
+int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                    const uint8_t *mat_ids, int method, sgm_plan **plan, sgm_stream_t stream);
+/* autotune (optional; uses the given arrays as scratch-copied input) */
+int sgm_plan_autotune(sgm_plan *plan, const double *rho, const double *vfrac,
+                      const double *temp, const double *press, const double *vfrac_tot,
+                      const double *sie_tot, const sgm_mix_params *params, int reps,
+                      sgm_stream_t stream);
+int sgm_plan_execute(sgm_plan *plan, double *rho, double *vfrac, double *sie, double *temp,
+                     double *press, const double *vfrac_tot, const double *sie_tot,
+                     const double *Tnorm, double *lambda, const sgm_mix_params *params,
+                     sgm_pte_status status, sgm_stream_t stream);
+/* one bin's kernel without the graph (per-kernel timing and profiling) */
+int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, double *sie,
+                         double *temp, double *press, const double *vfrac_tot,
+                         const double *sie_tot, const double *Tnorm, double *lambda,
+                         const sgm_mix_params *params, sgm_pte_status status,
+                         sgm_stream_t stream);
+/* host: per-bin description, out[6*i..] = {nmat, ncells, block, cells_per_wave, grid, start} */
+int sgm_plan_describe(const sgm_plan *plan, long *out, int max_bins);
+int sgm_plan_destroy(sgm_plan *plan);
+
+/* ---- defaults and misc -------------------------------------------------- */
+void sgm_mix_params_default(sgm_mix_params *p);
+/* MixParams of singularity-eos/example/pte_2mat.cpp:169-180 (1e-10 tolerances) */
+void sgm_mix_params_tight(sgm_mix_params *p);
+int sgm_device_info(int *ncu, int *arch_gfx, double *hbm_gbs);
+const char *sgm_version(void);
+
+/* ---- synthetic workloads (host memory; deterministic mt19937_64 streams) - */
+/* Gruneisen-sampled rho-T table on NQT log grids; fields as sgm_table_desc,
+ * caller-provided HOST buffers [nRho*nT] and [nRho]. */
+int sgm_build_gruneisen_table(double C0, double s1, double s2, double s3, double G0, double b,
+                              double rho0, double T0, double P0, double Cv, double lRhoMin,
+                              double lRhoMax, int nRho, double lTMin, double lTMax, int nT,
+                              double *P, double *sie, double *bMod, double *dPdRho,
+                              double *dPdE, double *dEdRho, double *dEdT, double *PCold,
+                              double *sieCold, double *bModCold);
+/* Two-material closure inputs (IdealGas A, StiffGas B), HOST arrays:
+ * rho/vfrac/sie/temp/press [2n], sie_tot [n]. */
+int sgm_workload_pte2(uint64_t seed, long n, double *rho, double *vfrac, double *sie,
+                      double *temp, double *press, double *sie_tot);
+/* Uneven closure inputs (2..6 materials from a pool of 6), HOST arrays sized
+ * for the worst case: offsets[n+1], mat_ids[6n], rho/vfrac/sie/temp/press[6n],
+ * sie_tot[n]. pool_matindex[6] maps pool slot -> context material index.
+ * Returns nnz. Requires the pool materials to be set on ctx (sie/press drawn
+ * from the context's EOS on the host). */
+long sgm_workload_uneven(sgm_ctx *ctx, uint64_t seed, long n, const int *pool_matindex,
+                         int *offsets, uint8_t *mat_ids, double *rho, double *vfrac,
+                         double *sie, double *temp, double *press, double *sie_tot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGM_H_ */

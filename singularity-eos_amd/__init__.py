@@ -1,0 +1,282 @@
+"""Python binding of libsgm.so — the C ABI declared in include/sgm.h.
+
+Device memory is passed as raw pointers (torch tensors' ``data_ptr()`` or
+ints); streams as ``torch.cuda.current_stream().cuda_stream`` or 0. This
+module is plumbing for tests and benchmarks: all computation happens in the
+HIP kernels of libsgm.so, and loading fails loudly when the library is absent.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgm.so")
+
+# EOS kinds / function selectors / thermalqs bits (include/sgm.h)
+IDEAL, STIFF, GRUNEISEN, VINET, TABLE_RHOT = 0, 1, 2, 3, 4
+F_P_RHO_SIE, F_T_RHO_SIE, F_SIE_RHO_T, F_P_RHO_T = 0, 1, 2, 3
+F_CV_RHO_SIE, F_CV_RHO_T, F_BMOD_RHO_SIE, F_BMOD_RHO_T, F_MIN_SIE_RHO = 4, 5, 6, 7, 8
+Q_DENSITY, Q_SIE, Q_PRESSURE, Q_TEMPERATURE, Q_CV, Q_BMOD = 1, 2, 4, 8, 16, 32
+ERRB_GRUN_MAXRHO, ERRB_GRUN_PT, ERRB_FILLEOS, ERRB_LINSOLVE, ERRB_SMALLSTEP, ERRB_INPUT = (
+    1, 2, 4, 8, 16, 32)
+
+
+class MixParams(C.Structure):
+    """sgm_mix_params (MixParams, mixed_cell_models.hpp:43-78)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "derivative_eps", "pte_rel_tolerance_p", "pte_rel_tolerance_t", "pte_rel_tolerance_e",
+        "pte_rel_tolerance_v", "pte_abs_tolerance_p", "pte_abs_tolerance_t",
+        "pte_slow_convergence_thresh", "pte_rel_tolerance_p_sufficient",
+        "pte_rel_tolerance_t_sufficient", "pte_rel_tolerance_e_sufficient",
+        "pte_rel_tolerance_v_sufficient", "pte_abs_tolerance_p_sufficient",
+        "pte_abs_tolerance_t_sufficient", "line_search_alpha", "line_search_fac",
+        "vfrac_safety_fac", "temperature_limit", "default_tguess", "min_dtde",
+        "pte_small_step_thresh", "pte_max_dpdv")] + [(n, C.c_int) for n in (
+        "pte_max_iter_per_mat", "line_search_max_iter", "pte_small_step_tries",
+        "pte_reduced_system_exclude_idx", "iterate_t_guess", "pad")]
+
+    # packed-double order used by the oracle's C entry points
+    ORACLE_ORDER = (
+        "derivative_eps", "pte_rel_tolerance_p", "pte_rel_tolerance_t", "pte_rel_tolerance_e",
+        "pte_rel_tolerance_v", "pte_abs_tolerance_p", "pte_abs_tolerance_t",
+        "pte_slow_convergence_thresh", "pte_rel_tolerance_p_sufficient",
+        "pte_rel_tolerance_t_sufficient", "pte_rel_tolerance_e_sufficient",
+        "pte_rel_tolerance_v_sufficient", "pte_abs_tolerance_p_sufficient",
+        "pte_abs_tolerance_t_sufficient", "pte_max_iter_per_mat", "line_search_alpha",
+        "line_search_max_iter", "line_search_fac", "vfrac_safety_fac", "temperature_limit",
+        "default_tguess", "min_dtde", "pte_small_step_tries", "pte_small_step_thresh",
+        "pte_max_dpdv", "pte_reduced_system_exclude_idx", "iterate_t_guess")
+
+    def as_oracle_array(self):
+        return [float(getattr(self, k)) for k in self.ORACLE_ORDER]
+
+
+class PteStatus(C.Structure):
+    _fields_ = [("converged", C.c_void_p), ("niter", C.c_void_p), ("residual", C.c_void_p),
+                ("err", C.c_void_p)]
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("lRhoMin", C.c_double), ("lRhoMax", C.c_double), ("nRho", C.c_int),
+                ("lTMin", C.c_double), ("lTMax", C.c_double), ("nT", C.c_int),
+                ("rho_offset", C.c_double), ("T_offset", C.c_double)] + [
+        (n, C.c_void_p) for n in ("P", "sie", "bMod", "dPdRho", "dPdE", "dEdRho", "dEdT",
+                                  "PCold", "sieCold", "bModCold")] + [
+        ("reproducible", C.c_int), ("pmin_vapor_dome", C.c_int)]
+
+
+_lib = None
+vp, ci, cl, cd = C.c_void_p, C.c_int, C.c_long, C.c_double
+
+_SIGS = {
+    "sgm_init": (ci, [ci, C.POINTER(vp)]),
+    "sgm_finalize": (ci, [vp]),
+    "sgm_set_IdealGas": (ci, [vp, ci, cd, cd]),
+    "sgm_set_StiffGas": (ci, [vp, ci, cd, cd, cd, cd]),
+    "sgm_set_Gruneisen": (ci, [vp, ci] + [cd] * 11),
+    "sgm_set_Vinet": (ci, [vp, ci] + [cd] * 8 + [vp]),
+    "sgm_set_TableRhoT": (ci, [vp, ci, C.POINTER(TableDesc)]),
+    "sgm_get_kind": (ci, [vp, ci]),
+    "sgm_table_query": (cl, [vp, ci, ci, vp]),
+    "sgm_vec_P_from_rho_sie": (ci, [vp, ci, vp, vp, vp, cl, ci, vp, vp]),
+    "sgm_vec_T_from_rho_sie": (ci, [vp, ci, vp, vp, vp, cl, ci, vp, vp]),
+    "sgm_vec_eval": (ci, [vp, ci, ci, vp, vp, vp, cl, ci, vp, vp]),
+    "sgm_vec_fill_eos": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, cl, C.c_ulong, ci, vp, vp, vp]),
+    "sgm_vec_rhoe_from_PT": (ci, [vp, ci, vp, vp, vp, vp, cl, ci, vp, vp, vp]),
+    "sgm_pte_rhoT": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                          C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_pte_PT": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                        C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_plan_create": (ci, [vp, cl, cl, vp, vp, ci, C.POINTER(vp), vp]),
+    "sgm_plan_autotune": (ci, [vp, vp, vp, vp, vp, vp, vp, C.POINTER(MixParams), ci, vp]),
+    "sgm_plan_execute": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(MixParams),
+                              PteStatus, vp]),
+    "sgm_plan_execute_bin": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                  C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_plan_describe": (ci, [vp, C.POINTER(C.c_long), ci]),
+    "sgm_plan_destroy": (ci, [vp]),
+    "sgm_mix_params_default": (None, [C.POINTER(MixParams)]),
+    "sgm_mix_params_tight": (None, [C.POINTER(MixParams)]),
+    "sgm_device_info": (ci, [C.POINTER(ci), C.POINTER(ci), C.POINTER(cd)]),
+    "sgm_version": (C.c_char_p, []),
+    "sgm_build_gruneisen_table": (ci, [cd] * 10 + [cd, cd, ci, cd, cd, ci] + [vp] * 10),
+    "sgm_workload_pte2": (ci, [C.c_uint64, cl, vp, vp, vp, vp, vp, vp]),
+    "sgm_workload_uneven": (cl, [vp, C.c_uint64, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+}
+
+
+def lib():
+    """Loads libsgm.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsgm.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def ptr(x):
+    """Device/host pointer of a torch tensor, numpy array, int or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    raise TypeError(type(x))
+
+
+def mix_params(tight=False, **over):
+    p = MixParams()
+    (lib().sgm_mix_params_tight if tight else lib().sgm_mix_params_default)(C.byref(p))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+def check(rc, what="sgm call"):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+    return rc
+
+
+class Context:
+    """A material registry on the current GPU (sgm_ctx)."""
+
+    def __init__(self, nmat):
+        self.h = vp()
+        check(lib().sgm_init(nmat, C.byref(self.h)), "sgm_init")
+        self.nmat = nmat
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().sgm_finalize(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # materials
+    def set_ideal(self, m, gm1, Cv):
+        return check(lib().sgm_set_IdealGas(self.h, m, gm1, Cv), "set_IdealGas")
+
+    def set_stiff(self, m, gm1, Cv, Pinf, qq):
+        return check(lib().sgm_set_StiffGas(self.h, m, gm1, Cv, Pinf, qq), "set_StiffGas")
+
+    def set_gruneisen(self, m, C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv, rho_max=0.0):
+        return check(lib().sgm_set_Gruneisen(self.h, m, C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv,
+                                             rho_max), "set_Gruneisen")
+
+    def set_vinet(self, m, rho0, T0, B0, BP0, A0, Cv0, E0, S0, d2to40=None):
+        arr = (C.c_double * 39)(*(d2to40 if d2to40 is not None else [0.0] * 39))
+        return check(lib().sgm_set_Vinet(self.h, m, rho0, T0, B0, BP0, A0, Cv0, E0, S0,
+                                         C.cast(arr, vp)), "set_Vinet")
+
+    def set_table(self, m, desc):
+        return check(lib().sgm_set_TableRhoT(self.h, m, C.byref(desc)), "set_TableRhoT")
+
+    def kind(self, m):
+        return lib().sgm_get_kind(self.h, m)
+
+    def table_query(self, m, which):
+        import numpy as np
+        n = lib().sgm_table_query(self.h, m, which, None)
+        if n < 0:
+            raise RuntimeError(f"table_query {n}")
+        out = np.zeros(n)
+        lib().sgm_table_query(self.h, m, which, out.ctypes.data)
+        return out
+
+    # vector calls
+    def vec_eval(self, m, func, in1, in2, out, n=None, lam=None, lam_stride=3, stream=0):
+        n = out.numel() if n is None else n
+        return lib().sgm_vec_eval(self.h, m, func, ptr(in1), ptr(in2), ptr(out), n,
+                                  lam_stride if lam is not None else 0, ptr(lam), stream)
+
+    def fill_eos(self, m, rho, temp, sie, press, cv, bmod, n, output, lam=None, lam_stride=3,
+                 errs=None, stream=0):
+        return lib().sgm_vec_fill_eos(self.h, m, ptr(rho), ptr(temp), ptr(sie), ptr(press),
+                                      ptr(cv), ptr(bmod), n, output,
+                                      lam_stride if lam is not None else 0, ptr(lam), ptr(errs),
+                                      stream)
+
+    def rhoe_from_PT(self, m, P, T, rho, sie, n, lam=None, lam_stride=3, errs=None, stream=0):
+        return lib().sgm_vec_rhoe_from_PT(self.h, m, ptr(P), ptr(T), ptr(rho), ptr(sie), n,
+                                          lam_stride if lam is not None else 0, ptr(lam),
+                                          ptr(errs), stream)
+
+    def pte(self, method, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press, vfrac_tot,
+            sie_tot, Tnorm=None, lam=None, params=None, conv=None, niter=None, resid=None,
+            err=None, stream=0):
+        st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
+        f = lib().sgm_pte_rhoT if method == 0 else lib().sgm_pte_PT
+        return f(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids), ptr(rho), ptr(vfrac), ptr(sie),
+                 ptr(temp), ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm), ptr(lam),
+                 C.byref(params) if params is not None else None, st, stream)
+
+
+class Plan:
+    """Batched persistent PTE plan over one CSR topology (sgm_plan)."""
+
+    def __init__(self, ctx, ncell, nnz, offsets, mat_ids, method=0, stream=0):
+        self.h = vp()
+        self.ctx = ctx
+        check(lib().sgm_plan_create(ctx.h, ncell, nnz, ptr(offsets), ptr(mat_ids), method,
+                                    C.byref(self.h), stream), "plan_create")
+
+    def autotune(self, rho, vfrac, temp, press, vfrac_tot, sie_tot, params=None, reps=1,
+                 stream=0):
+        return check(lib().sgm_plan_autotune(self.h, ptr(rho), ptr(vfrac), ptr(temp), ptr(press),
+                                             ptr(vfrac_tot), ptr(sie_tot),
+                                             C.byref(params) if params is not None else None,
+                                             reps, stream), "plan_autotune")
+
+    def execute(self, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm=None, lam=None,
+                params=None, conv=None, niter=None, resid=None, err=None, stream=0):
+        st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
+        return lib().sgm_plan_execute(self.h, ptr(rho), ptr(vfrac), ptr(sie), ptr(temp),
+                                      ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm),
+                                      ptr(lam), C.byref(params) if params is not None else None,
+                                      st, stream)
+
+    def execute_bin(self, b, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm=None,
+                    lam=None, params=None, conv=None, niter=None, resid=None, err=None,
+                    stream=0):
+        st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
+        return lib().sgm_plan_execute_bin(self.h, b, ptr(rho), ptr(vfrac), ptr(sie), ptr(temp),
+                                          ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm),
+                                          ptr(lam),
+                                          C.byref(params) if params is not None else None, st,
+                                          stream)
+
+    def describe(self):
+        buf = (C.c_long * (6 * 16))()
+        n = lib().sgm_plan_describe(self.h, buf, 16)
+        return [dict(nmat=buf[6 * i], ncells=buf[6 * i + 1], block=buf[6 * i + 2],
+                     cells_per_wave=buf[6 * i + 3], grid=buf[6 * i + 4], start=buf[6 * i + 5])
+                for i in range(min(n, 16))]
+
+    def close(self):
+        if self.h:
+            lib().sgm_plan_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,92 @@
+// Library-internal declarations shared by the C-ABI translation unit and the
+// kernel translation units.
+#pragma once
+
+#include "../../include/sgm.h"
+#include "sgm_pte.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace sgm {
+
+constexpr int MAX_MATS_PER_CELL = 8;
+constexpr int MAX_REGISTRY = 64;
+
+// Host-side ownership of one table: host arrays (for host-side evaluation and
+// queries) and the device copies the kernels read.
+struct TableHost {
+  std::vector<Real> P, sie, bMod, dEdT, dPdE;
+  std::vector<Real> PCold, sieCold, bModCold, dPdECold, dEdTCold, lTColdCrit;
+  std::vector<Real> PMax, sielTMax, dEdTMax, gm1Max, rho_at_pmin;
+  TableDev hdev{};       // pointers into the host vectors above
+  TableDev *d_tab = nullptr;  // device copy of the struct (pointers to device arrays)
+  std::vector<void *> d_arrays;
+};
+
+struct VinetHost {
+  std::vector<Real> coef; // d2..d40 (39) then VIP[0..42] (43)
+  Real *d_coef = nullptr;
+};
+
+} // namespace sgm
+
+struct sgm_ctx {
+  int device = 0;
+  int nmat = 0;
+  std::vector<sgm::EosDev> host_eos; // device-pointer view (what kernels get)
+  std::vector<sgm::EosDev> host_eval; // host-pointer view (for host-side evaluation)
+  std::vector<int> is_set;
+  std::vector<std::unique_ptr<sgm::TableHost>> tables;
+  std::vector<std::unique_ptr<sgm::VinetHost>> vinets;
+  sgm::EosDev *d_eos = nullptr;
+  bool dirty = true;
+  std::mutex mu;
+};
+
+namespace sgm {
+// launchers (kernels translation units)
+hipError_t launch_vec(const EosDev &e_host, const EosDev *d_eos, int mat, int func,
+                      const double *in1, const double *in2, double *out, long n, int lstride,
+                      double *lambda, hipStream_t s);
+hipError_t launch_fill(const EosDev &e_host, const EosDev *d_eos, int mat, double *rho,
+                       double *temp, double *sie, double *press, double *cv, double *bmod,
+                       long n, unsigned long output, int lstride, double *lambda, int *errs,
+                       hipStream_t s);
+hipError_t launch_rhoe_PT(const EosDev &e_host, const EosDev *d_eos, int mat, const double *P,
+                          const double *T, double *rho, double *sie, long n, int lstride,
+                          double *lambda, int *errs, hipStream_t s);
+
+struct PteArgs {
+  const EosDev *d_eos;
+  int nreg;
+  long ncell, nnz;
+  const int *offsets;
+  const uint8_t *mat_ids;
+  double *rho, *vfrac, *sie, *temp, *press;
+  const double *vfrac_tot, *sie_tot, *Tnorm;
+  double *lambda;
+  int *conv, *niter, *err;
+  double *resid;
+};
+hipError_t launch_pte_plain_rhot(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
+hipError_t launch_pte_plain_pt(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
+hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const int *cells,
+                                long n, hipStream_t s);
+hipError_t launch_pte_list_pt(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                              hipStream_t s);
+// batched bins (2..6 materials): list of cell ids with identical material count
+hipError_t launch_pte_bin_rhot(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
+                               long n, int *counter, int block, int cpw, int grid, hipStream_t s);
+hipError_t launch_pte_bin_pt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
+                             long n, int *counter, int block, int cpw, int grid, hipStream_t s);
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
+                            unsigned long long *keys, int *ids, hipStream_t s);
+int device_cu_count();
+// Gruneisen::ComputeRhoMax (eos_gruneisen.hpp:231-329), host side
+Real gruneisen_compute_rho_max(Real s1, Real s2, Real s3, Real rho0);
+} // namespace sgm

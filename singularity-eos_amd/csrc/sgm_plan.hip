@@ -1,0 +1,386 @@
+// Batched PTE plan: bins the cells of one CSR topology by material count and
+// material signature (radix sort of a 64-bit key), picks a launch shape per
+// bin (optionally by timing candidates on the device: the autotuner), and
+// captures one hipGraph holding a counter reset plus one persistent kernel per
+// bin. Replays of the graph solve every cell with a single graph launch.
+#include "sgm_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace sgm;
+
+namespace {
+struct Bin {
+  int nm;          // 2..6 register-resident kernels, 0 = generic list kernel
+  long start, n;   // range in the sorted cell list
+  int block = 256, cpw = 64, grid = 0;
+};
+
+struct GraphKey {
+  const void *ptrs[13];
+  MixParamsDev P;
+  bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+} // namespace
+
+struct sgm_plan {
+  sgm_ctx *ctx = nullptr;
+  int method = 0;
+  long ncell = 0, nnz = 0;
+  int *d_offsets = nullptr;
+  uint8_t *d_mat = nullptr;
+  int *d_sorted = nullptr;  // cell ids grouped by bin
+  int *d_counters = nullptr;
+  std::vector<Bin> bins;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool have_key = false;
+  GraphKey key{};
+};
+
+namespace {
+
+int occupancy_grid(int method, int nm, int block) {
+  // persistent grid: enough waves to fill every CU at the kernel's occupancy
+  // limit; register use grows with the material count
+  const int ncu = device_cu_count();
+  const int waves_per_cu_guess = nm <= 2 ? 16 : nm <= 3 ? 12 : nm <= 4 ? 8 : 6;
+  const int waves_per_block = block / 64;
+  int g = ncu * std::max(1, waves_per_cu_guess / waves_per_block);
+  (void)method;
+  return g;
+}
+
+hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
+  hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
+  if (e != hipSuccess) return e;
+  for (size_t i = 0; i < p->bins.size(); ++i) {
+    const Bin &b = p->bins[i];
+    const int *cells = p->d_sorted + b.start;
+    if (b.nm >= 2 && b.nm <= 6) {
+      e = p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
+                                               b.cpw, b.grid, s)
+                         : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
+                                             b.cpw, b.grid, s);
+    } else {
+      e = p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
+                         : launch_pte_list_pt(a, P, cells, b.n, s);
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+PteArgs make_args(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                  double *press, const double *vt, const double *st, const double *Tn,
+                  double *lambda, sgm_pte_status status) {
+  PteArgs a;
+  a.d_eos = p->ctx->d_eos;
+  a.nreg = p->ctx->nmat;
+  a.ncell = p->ncell;
+  a.nnz = p->nnz;
+  a.offsets = p->d_offsets;
+  a.mat_ids = p->d_mat;
+  a.rho = rho;
+  a.vfrac = vfrac;
+  a.sie = sie;
+  a.temp = temp;
+  a.press = press;
+  a.vfrac_tot = vt;
+  a.sie_tot = st;
+  a.Tnorm = Tn;
+  a.lambda = lambda;
+  a.conv = status.converged;
+  a.niter = status.niter;
+  a.err = status.err;
+  a.resid = status.residual;
+  return a;
+}
+
+MixParamsDev params_dev(const sgm_mix_params *params) {
+  sgm_mix_params q;
+  if (params)
+    q = *params;
+  else
+    sgm_mix_params_default(&q);
+  MixParamsDev d;
+  std::memcpy(&d, &q, sizeof(d));
+  return d;
+}
+
+} // namespace
+
+extern "C" {
+
+int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                    const uint8_t *mat_ids, int method, sgm_plan **out, sgm_stream_t stream) {
+  if (!ctx || !out || ncell < 1 || nnz < 1 || nnz > 2147483647L || ncell > 2147483647L ||
+      !offsets || !mat_ids || (method != 0 && method != 1))
+    return SGM_E_ARG;
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  auto *p = new sgm_plan();
+  p->ctx = ctx;
+  p->method = method;
+  p->ncell = ncell;
+  p->nnz = nnz;
+  bool ok = hipMalloc(&p->d_offsets, sizeof(int) * (ncell + 1)) == hipSuccess &&
+            hipMalloc(&p->d_mat, nnz) == hipSuccess &&
+            hipMalloc(&p->d_sorted, sizeof(int) * ncell) == hipSuccess;
+  unsigned long long *d_keys = nullptr, *d_keys_sorted = nullptr;
+  int *d_ids = nullptr;
+  void *d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  std::vector<unsigned long long> keys;
+  if (ok)
+    ok = hipMemcpyAsync(p->d_offsets, offsets, sizeof(int) * (ncell + 1),
+                        hipMemcpyDeviceToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(p->d_mat, mat_ids, nnz, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+         hipMalloc(&d_keys, sizeof(unsigned long long) * ncell) == hipSuccess &&
+         hipMalloc(&d_keys_sorted, sizeof(unsigned long long) * ncell) == hipSuccess &&
+         hipMalloc(&d_ids, sizeof(int) * ncell) == hipSuccess;
+  if (ok) ok = launch_plan_keys(p->d_offsets, p->d_mat, ncell, nnz, d_keys, d_ids, s) == hipSuccess;
+  if (ok)
+    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_ids,
+                                            p->d_sorted, (int)ncell, 0, 64, s) == hipSuccess &&
+         hipMalloc(&d_tmp, tmp_bytes + 16) == hipSuccess &&
+         hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys_sorted, d_ids,
+                                            p->d_sorted, (int)ncell, 0, 64, s) == hipSuccess;
+  if (ok) {
+    keys.resize(ncell);
+    ok = hipMemcpyAsync(keys.data(), d_keys_sorted, sizeof(unsigned long long) * ncell,
+                        hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+  }
+  if (d_keys) hipFree(d_keys);
+  if (d_keys_sorted) hipFree(d_keys_sorted);
+  if (d_ids) hipFree(d_ids);
+  if (d_tmp) hipFree(d_tmp);
+  if (!ok) {
+    sgm_plan_destroy(p);
+    return SGM_E_HIP;
+  }
+  // bins: one per material count 2..6; everything else in one generic list bin
+  long i = 0;
+  Bin other{0, -1, 0};
+  while (i < ncell) {
+    const int nm = (int)(keys[i] >> 56);
+    long j = i;
+    while (j < ncell && (int)(keys[j] >> 56) == nm) ++j;
+    if (nm >= 2 && nm <= 6) {
+      Bin b{nm, i, j - i};
+      b.block = 256;
+      b.cpw = 64;
+      b.grid = occupancy_grid(method, nm, b.block);
+      p->bins.push_back(b);
+    } else {
+      // sorted keys put counts 1 before 2..6 and 7, 8, invalid after; keep
+      // each such run as its own list bin
+      Bin b{0, i, j - i};
+      p->bins.push_back(b);
+    }
+    i = j;
+  }
+  (void)other;
+  if (hipMalloc(&p->d_counters, sizeof(int) * std::max<size_t>(1, p->bins.size())) !=
+          hipSuccess ||
+      hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+    sgm_plan_destroy(p);
+    return SGM_E_HIP;
+  }
+  *out = p;
+  return SGM_OK;
+}
+
+int sgm_plan_describe(const sgm_plan *p, long *out, int max_bins) {
+  if (!p || !out) return SGM_E_ARG;
+  int k = 0;
+  for (const Bin &b : p->bins) {
+    if (k >= max_bins) break;
+    out[6 * k + 0] = b.nm;
+    out[6 * k + 1] = b.n;
+    out[6 * k + 2] = b.block;
+    out[6 * k + 3] = b.cpw;
+    out[6 * k + 4] = b.grid;
+    out[6 * k + 5] = b.start;
+    ++k;
+  }
+  return (int)p->bins.size();
+}
+
+// Times each candidate launch shape for every register-resident bin on
+// scratch copies of the inputs and keeps the fastest. Results of the solve do
+// not depend on the launch shape (cells are independent), so any choice is
+// exact; only throughput changes.
+int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const double *temp,
+                      const double *press, const double *vfrac_tot, const double *sie_tot,
+                      const sgm_mix_params *params, int reps, sgm_stream_t stream) {
+  if (!p || !rho || !vfrac || !temp || !press || !vfrac_tot || !sie_tot) return SGM_E_ARG;
+  if (reps < 1) reps = 1;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nb = sizeof(double) * p->nnz;
+  double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (auto &q : w)
+    if (hipMalloc(&q, nb) != hipSuccess) return SGM_E_HIP;
+  const MixParamsDev P = params_dev(params);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int ncu = device_cu_count();
+  int rc = SGM_OK;
+  for (size_t bi = 0; bi < p->bins.size() && rc == SGM_OK; ++bi) {
+    Bin &b = p->bins[bi];
+    if (b.nm < 2 || b.nm > 6 || b.n < 64) continue;
+    const int blocks[3] = {64, 128, 256};
+    const int cpws[3] = {64, 128, 256};
+    const int waves_per_cu[4] = {2, 4, 8, 16};
+    float best = 1e30f;
+    Bin bestb = b;
+    for (int bl : blocks)
+      for (int cpw : cpws)
+        for (int wpc : waves_per_cu) {
+          const int wpb = bl / 64;
+          const int grid = std::max(1, ncu * wpc / wpb);
+          float tot = 0;
+          for (int r = 0; r < reps && rc == SGM_OK; ++r) {
+            if (hipMemcpyAsync(w[0], rho, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(w[1], vfrac, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(w[2], temp, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(w[3], press, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                launch_zero_ints(p->d_counters + bi, 1, s) != hipSuccess) {
+              rc = SGM_E_HIP;
+              break;
+            }
+            PteArgs a = make_args(p, w[0], w[1], w[4], w[2], w[3], vfrac_tot, sie_tot, nullptr,
+                                  nullptr, sgm_pte_status{nullptr, nullptr, nullptr, nullptr});
+            hipEventRecord(e0, s);
+            const hipError_t e =
+                p->method == 0
+                    ? launch_pte_bin_rhot(b.nm, a, P, p->d_sorted + b.start, b.n,
+                                          p->d_counters + bi, bl, cpw, grid, s)
+                    : launch_pte_bin_pt(b.nm, a, P, p->d_sorted + b.start, b.n,
+                                        p->d_counters + bi, bl, cpw, grid, s);
+            hipEventRecord(e1, s);
+            if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+              rc = SGM_E_HIP;
+              break;
+            }
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            tot += ms;
+          }
+          if (rc == SGM_OK && tot < best) {
+            best = tot;
+            bestb.block = bl;
+            bestb.cpw = cpw;
+            bestb.grid = grid;
+          }
+        }
+    b = bestb;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  for (auto &q : w) hipFree(q);
+  // launch shapes changed: the captured graph is stale
+  if (p->exec) {
+    hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->have_key = false;
+  return rc;
+}
+
+int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                     double *press, const double *vfrac_tot, const double *sie_tot,
+                     const double *Tnorm, double *lambda, const sgm_mix_params *params,
+                     sgm_pte_status status, sgm_stream_t stream) {
+  if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot) return SGM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const MixParamsDev P = params_dev(params);
+  GraphKey k;
+  std::memset(&k, 0, sizeof(k));
+  const void *ptrs[13] = {rho,    vfrac,          sie,         temp, status.residual,
+                          press,  vfrac_tot,      sie_tot,     Tnorm,
+                          lambda, status.converged, status.niter, status.err};
+  for (int i = 0; i < 13; ++i) k.ptrs[i] = ptrs[i];
+  k.P = P;
+  // status.residual participates through the captured kernel arguments
+  if (!p->have_key || !(p->key == k) || !p->exec) {
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+    PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
+                          status);
+    if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return SGM_E_HIP;
+    const hipError_t le = launch_bins(p, a, P, p->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(p->cap_stream, &g);
+    if (le != hipSuccess || ce != hipSuccess || !g) {
+      if (g) hipGraphDestroy(g);
+      return SGM_E_HIP;
+    }
+    if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      hipGraphDestroy(g);
+      return SGM_E_HIP;
+    }
+    p->graph = g;
+    p->key = k;
+    p->have_key = true;
+  }
+  return hipGraphLaunch(p->exec, s) == hipSuccess ? SGM_OK : SGM_E_HIP;
+}
+
+// Launches one bin's kernel directly (no graph) — for per-kernel timing.
+int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, double *sie,
+                         double *temp, double *press, const double *vfrac_tot,
+                         const double *sie_tot, const double *Tnorm, double *lambda,
+                         const sgm_mix_params *params, sgm_pte_status status,
+                         sgm_stream_t stream) {
+  if (!p || bin < 0 || bin >= (int)p->bins.size() || !rho || !vfrac || !sie || !temp ||
+      !press || !vfrac_tot || !sie_tot)
+    return SGM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const MixParamsDev P = params_dev(params);
+  PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
+                        status);
+  const Bin &b = p->bins[bin];
+  hipError_t e = launch_zero_ints(p->d_counters + bin, 1, s);
+  if (e != hipSuccess) return SGM_E_HIP;
+  const int *cells = p->d_sorted + b.start;
+  if (b.nm >= 2 && b.nm <= 6)
+    e = p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + bin,
+                                             b.block, b.cpw, b.grid, s)
+                       : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + bin, b.block,
+                                           b.cpw, b.grid, s);
+  else
+    e = p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
+                       : launch_pte_list_pt(a, P, cells, b.n, s);
+  return e == hipSuccess ? SGM_OK : SGM_E_HIP;
+}
+
+int sgm_plan_destroy(sgm_plan *p) {
+  if (!p) return SGM_E_ARG;
+  if (p->cap_stream) hipStreamSynchronize(p->cap_stream);
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->d_offsets) hipFree(p->d_offsets);
+  if (p->d_mat) hipFree(p->d_mat);
+  if (p->d_sorted) hipFree(p->d_sorted);
+  if (p->d_counters) hipFree(p->d_counters);
+  if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+  delete p;
+  return SGM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,784 @@
+// Register-resident pressure-temperature-equilibrium closure for one mixed
+// cell per lane, with the material count NM fixed at compile time so every
+// per-material array lives in VGPRs. Restates
+// singularity-eos/closure/mixed_cell_models.hpp (PTESolverBase :329-786,
+// PTESolverRhoT :873-1193, PTESolverPT :1198-1446, PTESolver :2188-2291) in
+// the same operation order. Indices that the reference computes at run time
+// (the excluded material `ms`, column pivots) are resolved with unrolled
+// selects instead of dynamic register indexing.
+#pragma once
+
+#include "sgm_eos.hpp"
+
+namespace sgm {
+
+// MixParams (mixed_cell_models.hpp:43-78) as a flat POD for kernel arguments
+struct MixParamsDev {
+  Real derivative_eps;
+  Real pte_rel_tolerance_p, pte_rel_tolerance_t, pte_rel_tolerance_e, pte_rel_tolerance_v;
+  Real pte_abs_tolerance_p, pte_abs_tolerance_t;
+  Real pte_slow_convergence_thresh;
+  Real pte_rel_tolerance_p_sufficient, pte_rel_tolerance_t_sufficient;
+  Real pte_rel_tolerance_e_sufficient, pte_rel_tolerance_v_sufficient;
+  Real pte_abs_tolerance_p_sufficient, pte_abs_tolerance_t_sufficient;
+  Real line_search_alpha, line_search_fac, vfrac_safety_fac, temperature_limit;
+  Real default_tguess, min_dtde, pte_small_step_thresh, pte_max_dpdv;
+  int pte_max_iter_per_mat, line_search_max_iter, pte_small_step_tries;
+  int pte_reduced_system_exclude_idx, iterate_t_guess, pad;
+};
+
+struct CellStatus {
+  int converged;
+  int niter;
+  int err;
+  Real residual;
+};
+
+// --- unrolled selects -------------------------------------------------------
+template <int N>
+SGM_HD Real pick(const Real (&a)[N], int i) {
+  Real r = a[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k)
+    if (i == k) r = a[k];
+  return r;
+}
+template <int N>
+SGM_HD void put(Real (&a)[N], int i, Real v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (i == k) a[k] = v;
+}
+
+// --- small dense solve (mixed_cell_models.hpp:176-278) ----------------------
+// Column-pivoted Householder QR for N >= 3 (the reference's Eigen
+// colPivHouseholderQr). Pivot on the largest updated column norm (first on
+// ties), LAPACK-style norm downdating with a sqrt(eps) recompute threshold,
+// beta = -sign(c0)||x||, pivots with squared norm below (max*eps)^2/N*(N-k)
+// treated as zero.
+template <int N>
+SGM_HD void qr_colpiv_solve(Real (&a)[N * N], Real (&b)[N], Real (&x)[N]) {
+  int perm[N];
+  Real cu[N], cd[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    perm[j] = j;
+    Real s = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += a[i * N + j] * a[i * N + j];
+    cd[j] = sqrt(s);
+    cu[j] = cd[j];
+  }
+  const Real eps = 2.220446049250313e-16;
+  const Real norm_downdate_threshold = sqrt(eps);
+  int nzp = N;
+  Real th = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int big = k;
+    Real bign = cu[k];
+#pragma unroll
+    for (int j = k + 1; j < N; ++j)
+      if (cu[j] > bign) {
+        bign = cu[j];
+        big = j;
+      }
+    const Real big_sq = bign * bign;
+    if (k == 0) th = big_sq * eps * eps / N;
+    if (nzp == N && big_sq < th * (N - k)) nzp = k;
+#pragma unroll
+    for (int j = k + 1; j < N; ++j) {
+      if (big == j) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const Real t = a[i * N + k];
+          a[i * N + k] = a[i * N + j];
+          a[i * N + j] = t;
+        }
+        const int tp = perm[k];
+        perm[k] = perm[j];
+        perm[j] = tp;
+        Real t = cu[k];
+        cu[k] = cu[j];
+        cu[j] = t;
+        t = cd[k];
+        cd[k] = cd[j];
+        cd[j] = t;
+      }
+    }
+    Real tailsq = 0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) tailsq += a[i * N + k] * a[i * N + k];
+    const Real c0 = a[k * N + k];
+    Real tau, beta;
+    if (tailsq <= 2.2250738585072014e-308) {
+      tau = 0;
+      beta = c0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) a[i * N + k] = 0;
+    } else {
+      beta = sqrt(c0 * c0 + tailsq);
+      if (c0 >= 0) beta = -beta;
+      const Real den = c0 - beta;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) a[i * N + k] = a[i * N + k] / den;
+      tau = (beta - c0) / beta;
+    }
+    a[k * N + k] = beta;
+#pragma unroll
+    for (int j = k + 1; j < N; ++j) {
+      Real s = 0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) s += a[i * N + k] * a[i * N + j];
+      s += a[k * N + j];
+      a[k * N + j] -= tau * s;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) a[i * N + j] -= (tau * a[i * N + k]) * s;
+    }
+    {
+      Real s = 0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) s += a[i * N + k] * b[i];
+      s += b[k];
+      b[k] -= tau * s;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) b[i] -= (tau * a[i * N + k]) * s;
+    }
+#pragma unroll
+    for (int j = k + 1; j < N; ++j) {
+      if (cu[j] != 0) {
+        Real t = fabs(a[k * N + j]) / cu[j];
+        t = (1 + t) * (1 - t);
+        if (t < 0) t = 0;
+        const Real r = cu[j] / cd[j];
+        const Real t2 = t * r * r;
+        if (t2 <= norm_downdate_threshold) {
+          Real s = 0;
+#pragma unroll
+          for (int i = k + 1; i < N; ++i) s += a[i * N + j] * a[i * N + j];
+          cd[j] = sqrt(s);
+          cu[j] = cd[j];
+        } else {
+          cu[j] *= sqrt(t);
+        }
+      }
+    }
+  }
+  Real z[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) z[i] = b[i];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    if (i < nzp) {
+      z[i] = z[i] / a[i * N + i];
+#pragma unroll
+      for (int r = 0; r < i; ++r) z[r] -= z[i] * a[r * N + i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) x[j] = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const Real zi = (i < nzp) ? z[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (perm[i] == j) x[j] = zi;
+  }
+}
+
+SGM_HD bool isfinite_ref(Real a) { return (a == a) && ((a == 0) || (a != 2 * a)); }
+
+// solve_Ax_b_wscr: Jacobi row scaling, then direct (N=1), Cramer (N=2) or
+// pivoted QR (N>=3). b is overwritten with x. false on a zero row or non-finite x.
+template <int N>
+SGM_HD bool solve_small(Real (&a)[N * N], Real (&b)[N]) {
+#pragma unroll
+  for (int row = 0; row < N; ++row) {
+    Real maxabs = 0;
+#pragma unroll
+    for (int c = 0; c < N; ++c) maxabs = smax(fabs(a[row * N + c]), maxabs);
+    if (maxabs == 0) return false;
+#pragma unroll
+    for (int c = 0; c < N; ++c) a[row * N + c] /= maxabs;
+    b[row] /= maxabs;
+  }
+  if constexpr (N == 1) {
+    b[0] = robust::ratio(b[0], a[0]);
+  } else if constexpr (N == 2) {
+    const Real det = a[0] * a[3] - a[1] * a[2];
+    const Real x0 = robust::ratio(a[3] * b[0] - a[1] * b[1], det);
+    const Real x1 = robust::ratio(a[0] * b[1] - a[2] * b[0], det);
+    b[0] = x0;
+    b[1] = x1;
+  } else {
+    Real x[N];
+    qr_colpiv_solve<N>(a, b, x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] = x[i];
+  }
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (!isfinite_ref(b[i])) ok = false;
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Shared base state (PTESolverBase)
+// ---------------------------------------------------------------------------
+template <int NM, int NEQ>
+struct PteBase {
+  const EosDev *eos[NM];
+  Lam lam[NM];
+  Real rho[NM], vfrac[NM], sie[NM], press[NM], u[NM], rhobar[NM];
+  Real jac[NEQ * NEQ], dx[NEQ], res[NEQ];
+  Real vfrac_total, sie_total, rho_total, uscale, utotal_scale, Tnorm;
+  int err;
+  MixParamsDev P;
+
+  // :388-399
+  SGM_HD void InitRhoBarandRho() {
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      rhobar[m] = rho[m] * vfrac[m];
+      s.add(rhobar[m]);
+    }
+    rho_total = s.result();
+  }
+  SGM_HD void NormalizeVfrac() {
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) s.add(vfrac[m]);
+    const Real f = robust::ratio(vfrac_total, s.result());
+#pragma unroll
+    for (int m = 0; m < NM; ++m) vfrac[m] *= f;
+  }
+  // :423-498; returns true on FAIL
+  SGM_HD bool SetVfracFromT(Real T) {
+    constexpr Real min_dv = 1e-12;
+    constexpr Real safety = 0.9;
+    Real vmax[NM];
+    Real vmax_sum = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      vmax[m] = smin(safety * robust::ratio(rhobar[m], eos_rho_pmin(*eos[m], T)), 1.0);
+      if (vfrac[m] <= 0) vfrac[m] = vmax[m];
+      vmax_sum += vmax[m];
+    }
+    if (vmax_sum < vfrac_total) return true;
+    NormalizeVfrac();
+    for (int it = 0; it < NM; ++it) {
+      Real remaining = 0;
+      int n_uncapped = NM;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const Real vnew = smin(vmax[m], vfrac[m]);
+        const Real dv = vfrac[m] - vnew;
+        vfrac[m] -= dv;
+        remaining += dv;
+        n_uncapped -= (dv > min_dv);
+      }
+      if (fabs(remaining) < min_dv) break;
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+        // ratio(Real, size_t) in the reference adds SMALL<size_t>() == 0: a plain division
+        if ((vmax[m] - vfrac[m]) > min_dv) vfrac[m] += remaining / (Real)n_uncapped;
+    }
+    NormalizeVfrac();
+    return false;
+  }
+  // :583-597
+  SGM_HD Real PressureFromPreferred(int m, Real r, Real T, Real s) {
+    const EosDev &e = *eos[m];
+    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
+    return eos_P_rT(e, r, T, lam[m]);
+  }
+  // :599-620
+  SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
+    const EosDev &e = *eos[m];
+    if (e.kind == KIND_IDEAL) {
+      // IdealGas is the only model here whose preferred input includes
+      // pressure: MinimumPressure() = 0, MaximumPressureAtTemperature() = 1e100
+      // (eos_base.hpp:544-549), closed-form P-T inversion (eos_ideal.hpp:201-207)
+      const Real Pmin = 0.0, Pmax = 1e100;
+      const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
+      ideal::rhoe_PT(e.p, Pguess, T, r, s);
+      cv = e.p[0];
+    } else {
+      s = eos_e_rT(e, r, T, lam[m]);
+      cv = eos_cv_rT(e, r, T, lam[m]);
+    }
+  }
+  // :500-581; Tin are the caller's input temperatures
+  SGM_HD Real GetTguess(const Real (&Tin)[NM]) {
+    Real Tguess;
+    if (Tnorm > 0.0) {
+      Tguess = Tnorm;
+    } else {
+      Tguess = P.default_tguess;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) Tguess = smax(Tguess, Tin[m]);
+    }
+    bool bad = false;
+    for (int i = 0; i < 3; i++) {
+      if (P.iterate_t_guess) {
+        Real dudt = 0, usum = 0;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+          const Real rho_max = eos_max_density(*eos[m]);
+          Real s, cv;
+          SieCvFromTAndPreferred(m, smin(rho[m], rho_max), press[m], Tguess, s, cv);
+          usum += rhobar[m] * s;
+          dudt += rhobar[m] * cv;
+        }
+        const Real Tn = Tguess - robust::ratio((usum - utotal_scale * uscale), dudt);
+        Tguess = smin(P.temperature_limit, smax(Tguess, Tn));
+      }
+#pragma unroll
+      for (int m = 0; m < NM; ++m) Tguess = smax(eos_min_temperature(*eos[m]), Tguess);
+      bad = SetVfracFromT(Tguess);
+      if (bad) {
+        Tguess *= 10.0;
+        continue;
+      }
+#pragma unroll
+      for (int m = 0; m < NM; ++m) rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      bad = false;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        if (!bad && rho[m] < eos_rho_pmin(*eos[m], Tguess)) {
+          bad = true;
+          Tguess *= 10.0;
+        }
+      }
+      if (!bad) break;
+    }
+    return Tguess;
+  }
+  // :627-662
+  SGM_HD void InitBase(const Real (&Tin)[NM]) {
+    InitRhoBarandRho();
+    const Real utotal = rho_total * sie_total;
+    uscale = smax(fabs(utotal), 1.0e-14);
+    utotal_scale = robust::ratio(utotal, uscale);
+    const Real Tguess = GetTguess(Tin);
+    Tnorm = Tguess;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      sie[m] = eos_e_rT(*eos[m], rho[m], Tguess, lam[m]);
+      if ((preferred_input(*eos[m]) & tq::pressure) && !bad_value(press[m])) {
+        press[m] = robust::ratio(press[m], uscale);
+      } else {
+        press[m] = robust::ratio(PressureFromPreferred(m, rho[m], Tguess, sie[m]), uscale);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) u[m] = sie[m] * robust::ratio(rhobar[m], uscale);
+  }
+  SGM_HD Real ResidualNorm() const {
+    NeumaierSum s;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) s.add(res[i] * res[i]);
+    return 0.5 * s.result();
+  }
+  SGM_HD bool Solve() {
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) dx[i] = res[i];
+    return solve_small<NEQ>(jac, dx);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverRhoT (:873-1193). Unknowns (T, alpha_m for m != ms).
+// ---------------------------------------------------------------------------
+template <int NM>
+struct PteRhoT : PteBase<NM, NM> {
+  using B = PteBase<NM, NM>;
+  using B::dx;
+  using B::eos;
+  using B::jac;
+  using B::lam;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::utotal_scale;
+  using B::vfrac;
+  Real vtemp[NM];
+  Real Tequil, Ttemp;
+  int ms;
+
+  // material index of reduced-system row/column k (1..NM-1)
+  SGM_HD int mat_of(int k) const { return (k <= ms) ? k - 1 : k; }
+
+  SGM_HD void Residual() {
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) s.add(u[m]);
+    res[0] = utotal_scale - s.result();
+    const Real pms = pick<NM>(press, ms);
+#pragma unroll
+    for (int k = 1; k < NM; ++k) {
+      const Real pm = (k <= ms) ? press[k - 1] : press[k];
+      res[k] = pms - pm;
+    }
+  }
+  // :947-984
+  SGM_HD Real Init(const Real (&Tin)[NM]) {
+    B::InitBase(Tin);
+    if (P.pte_reduced_system_exclude_idx >= 0) {
+      ms = P.pte_reduced_system_exclude_idx;
+    } else {
+      ms = 0;
+      Real xmax = 0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const Real x = robust::ratio(rhobar[m], B::rho_total);
+        if (x > xmax) {
+          xmax = x;
+          ms = m;
+        }
+      }
+    }
+    Residual();
+    Tequil = 1.0;
+    return B::ResidualNorm();
+  }
+  // :986-1008
+  SGM_HD void Fixup() {
+    if (P.pte_reduced_system_exclude_idx < 0) {
+      Real amax = 0;
+      ms = 0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+        if (vfrac[m] > amax) {
+          amax = vfrac[m];
+          ms = m;
+        }
+      Residual();
+    }
+  }
+  // :1021-1040 -> (converged, acceptable)
+  SGM_HD void CheckPTE(bool &conv, bool &ok) const {
+    Real mean_p = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mean_p += vfrac[m] * press[m];
+    NeumaierSum s;
+#pragma unroll
+    for (int k = 1; k < NM; ++k) s.add(res[k] * res[k]);
+    const Real error_p = sqrt(s.result());
+    const Real error_u = fabs(res[0]);
+    const Real amp = fabs(mean_p);
+    const bool cp = (error_p < P.pte_rel_tolerance_p * amp || error_p < P.pte_abs_tolerance_p);
+    const bool cu = (error_u < P.pte_rel_tolerance_e);
+    const bool ap = (error_p < P.pte_rel_tolerance_p_sufficient * amp ||
+                     error_p < P.pte_abs_tolerance_p_sufficient);
+    const bool au = (error_u < P.pte_rel_tolerance_e_sufficient);
+    conv = cp && cu;
+    ok = ap && au;
+  }
+  // :1042-1105
+  SGM_HD void Jacobian() {
+    Real dpdv[NM], dedv[NM], dpdT[NM];
+    Real dedT_sum = 0.0;
+    const Real Tcur = Tnorm * Tequil;
+    const Real dT = Tequil * P.derivative_eps;
+    const Real Tpert = Tnorm * (Tequil + dT);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      Real e_pert = eos_e_rT(*eos[m], rho_pert, Tcur, lam[m]);
+      Real p_pert = robust::ratio(B::PressureFromPreferred(m, rho_pert, Tcur, e_pert), uscale);
+      dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
+      dedv[m] = robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dv);
+      e_pert = eos_e_rT(*eos[m], rho[m], Tpert, lam[m]);
+      p_pert = robust::ratio(B::PressureFromPreferred(m, rho[m], Tpert, e_pert), uscale);
+      dpdT[m] = robust::ratio((p_pert - press[m]), dT);
+      dedT_sum += robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dT);
+    }
+    const Real dedv_s = pick<NM>(dedv, ms), dpdT_s = pick<NM>(dpdT, ms),
+               dpdv_s = pick<NM>(dpdv, ms);
+    jac[0] = dedT_sum;
+#pragma unroll
+    for (int k = 1; k < NM; ++k) {
+      const Real dedv_k = (k <= ms) ? dedv[k - 1] : dedv[k];
+      jac[k] = dedv_k - dedv_s;
+    }
+#pragma unroll
+    for (int r = 1; r < NM; ++r) {
+      const Real dpdT_r = (r <= ms) ? dpdT[r - 1] : dpdT[r];
+      const Real dpdv_r = (r <= ms) ? dpdv[r - 1] : dpdv[r];
+      jac[r * NM + 0] = dpdT_r - dpdT_s;
+#pragma unroll
+      for (int c = 1; c < NM; ++c) jac[r * NM + c] = (c == r) ? dpdv_r + dpdv_s : dpdv_s;
+    }
+  }
+  // :1107-1151 (including the reference's idx bookkeeping, which does not
+  // advance for materials skipped in the second pass)
+  SGM_HD Real ScaleDx() {
+    Real scale = 1.0;
+    NeumaierSum s;
+#pragma unroll
+    for (int i = 0; i < NM - 1; ++i) s.add(dx[1 + i]);
+    const Real dalphaskip = -s.result();
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real mydx = (m == ms) ? dalphaskip : ((m < ms) ? dx[m + 1] : dx[m]);
+      if (scale * mydx < -P.vfrac_safety_fac * vfrac[m])
+        scale = smin(scale, fabs(P.vfrac_safety_fac * robust::ratio(vfrac[m], mydx)));
+    }
+    const Real Tnew = Tequil + scale * dx[0];
+    int idx = 1;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real rho_min =
+          smax(eos_rho_pmin(*eos[m], Tnorm * Tequil), eos_rho_pmin(*eos[m], Tnorm * Tnew));
+      const Real alpha_max = smin(1.0, robust::ratio(rhobar[m], rho_min));
+      if (!(alpha_max < vfrac[m])) {
+        Real mydx;
+        if (m == ms) {
+          mydx = dalphaskip;
+        } else {
+          mydx = pick<NM>(dx, idx);
+          idx++;
+        }
+        if (scale * mydx > 0.5 * (alpha_max - vfrac[m]))
+          scale = smin(robust::ratio(0.5 * (alpha_max - vfrac[m]), mydx), scale);
+      }
+    }
+    if (scale * dx[0] < -0.95 * Tequil) scale = smin(scale, robust::ratio(-0.95 * Tequil, dx[0]));
+#pragma unroll
+    for (int i = 0; i < NM; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :1155-1187
+  SGM_HD Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+      Ttemp = Tequil;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) vtemp[m] = vfrac[m];
+    }
+    Tequil = Ttemp + scale * dx[0];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real dxm = (m < ms) ? dx[m + 1] : dx[m];
+      if (m != ms) vfrac[m] = vtemp[m] + scale * dxm;
+    }
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      if (m != ms) s.add(vfrac[m]);
+    put<NM>(vfrac, ms, 1 - s.result());
+    const Real Tc = Tnorm * Tequil;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      Real um = rhobar[m] * eos_e_rT(*eos[m], rho[m], Tc, lam[m]);
+      sie[m] = robust::ratio(um, rhobar[m]);
+      u[m] = robust::ratio(um, uscale);
+      press[m] = robust::ratio(B::PressureFromPreferred(m, rho[m], Tc, sie[m]), uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  SGM_HD Real Pout(int m) const { return press[m] * uscale; }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverPT (:1198-1446). Unknowns (T, P); 2x2 system.
+// ---------------------------------------------------------------------------
+template <int NM>
+struct PtePT : PteBase<NM, 2> {
+  using B = PteBase<NM, 2>;
+  using B::dx;
+  using B::eos;
+  using B::err;
+  using B::jac;
+  using B::lam;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::utotal_scale;
+  using B::vfrac;
+  using B::vfrac_total;
+  Real Pequil, Tequil, Ptemp, Ttemp;
+
+  SGM_HD void Residual() {
+    NeumaierSum sv, se;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) sv.add(vfrac[m]);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) se.add(u[m]);
+    res[0] = vfrac_total - sv.result();
+    res[1] = utotal_scale - se.result();
+  }
+  // :1258-1297
+  SGM_HD Real Init(const Real (&Tin)[NM]) {
+    B::InitBase(Tin);
+    Residual();
+    Pequil = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) Pequil += fabs(press[m]) * vfrac[m];
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) s.add(vfrac[m]);
+    Pequil /= s.result();
+    Tequil = 1;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      eos_rhoe_PT(*eos[m], Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      vfrac[m] = robust::ratio(rhobar[m], rho[m]);
+      u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD void Fixup() {}
+  SGM_HD void CheckPTE(bool &conv, bool &ok) const {
+    const Real ev = fabs(res[0]), eu = fabs(res[1]);
+    conv = (ev < P.pte_rel_tolerance_v) && (eu < P.pte_rel_tolerance_e);
+    ok = (eu < P.pte_rel_tolerance_e_sufficient) && (ev < P.pte_rel_tolerance_v_sufficient);
+  }
+  // :1336-1385
+  SGM_HD void Jacobian() {
+    Real dudT = 0.0, dudP = 0.0, rbT = 0.0, rbP = 0.0;
+    const Real dp = -Pequil * P.derivative_eps;
+    const Real dT = Tequil * P.derivative_eps;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      Real r_pert = rho[m];
+      Real e_pert = sie[m];
+      eos_rhoe_PT(*eos[m], uscale * (Pequil + dp), Tnorm * Tequil, r_pert, e_pert, lam[m], err);
+      const Real drdp = robust::ratio(r_pert - rho[m], dp);
+      const Real dudp = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dp);
+      rbP += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdp;
+      dudP += dudp;
+      eos_rhoe_PT(*eos[m], uscale * Pequil, Tnorm * (Tequil + dT), r_pert, e_pert, lam[m], err);
+      const Real drdT = robust::ratio(r_pert - rho[m], dT);
+      const Real dudTm = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dT);
+      rbT += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdT;
+      dudT += dudTm;
+    }
+    jac[0] = -rbT;
+    jac[1] = -rbP;
+    jac[2] = dudT;
+    jac[3] = dudP;
+  }
+  // :1387-1409
+  SGM_HD Real ScaleDx() {
+    Real scale = 1.0;
+    if (scale * dx[0] < -0.95 * Tequil) scale = robust::ratio(-0.95 * Tequil, dx[0]);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real Pmin = eos_min_pressure(*eos[m]);
+      scale = smin(fabs(scale),
+                   fabs(0.95 * robust::ratio(robust::ratio(Pmin, uscale) - Pequil, dx[1])));
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real Ttest = (Tequil + scale * dx[0]) * Tnorm;
+      const Real Pmax = eos_max_pressure_at_T(*eos[m], Ttest, err);
+      scale = smin(fabs(scale),
+                   fabs(0.95 * robust::ratio(robust::ratio(Pmax, uscale) - Pequil, dx[0])));
+    }
+    dx[0] *= scale;
+    dx[1] *= scale;
+    return scale;
+  }
+  // :1413-1435
+  SGM_HD Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+      Ttemp = Tequil;
+      Ptemp = Pequil;
+    }
+    Tequil = Ttemp + scale * dx[0];
+    Pequil = Ptemp + scale * dx[1];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      eos_rhoe_PT(*eos[m], Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      vfrac[m] = robust::ratio(rhobar[m], rho[m]);
+      u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  SGM_HD Real Pout(int) const { return Pequil * uscale; }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolver driver (:2188-2291)
+// ---------------------------------------------------------------------------
+template <int NM, class S>
+SGM_HD CellStatus pte_solve(S &s, const Real (&Tin)[NM]) {
+  const MixParamsDev &p = s.P;
+  CellStatus st;
+  st.converged = 0;
+  st.niter = 0;
+  Real err = s.Init(Tin);
+  bool converged = false, close_enough = false;
+  const int max_iter = NM * p.pte_max_iter_per_mat;
+  int small_step_iters = 0;
+  int niter;
+  for (niter = 0; niter < max_iter; ++niter) {
+    st.niter = niter;
+    s.CheckPTE(converged, close_enough);
+    if (converged) break;
+    s.Jacobian();
+    if (!s.Solve()) {
+      converged = false;
+      s.err |= ERRB_LINSOLVE;
+      break;
+    }
+    Real scale = smin(1.0, s.ScaleDx());
+    if (fabs(scale) < p.pte_small_step_thresh) small_step_iters++;
+    if (small_step_iters >= p.pte_small_step_tries) {
+      converged = false;
+      s.err |= ERRB_SMALLSTEP;
+      break;
+    }
+    const Real gradfdx = -2.0 * scale * err;
+    scale = 1.0;
+    const Real err_old = err;
+    err = s.TestUpdate(scale, true);
+    if (err > err_old + p.line_search_alpha * gradfdx) {
+      scale = 0.5;
+      const Real err_mid = s.TestUpdate(scale, false);
+      if (err_mid < err && err_mid < err_old)
+        scale = 0.75 + 0.5 * robust::ratio(err_mid - err, err - 2.0 * err_mid + err_old);
+      for (int li = 0; li < p.line_search_max_iter; li++) {
+        err = s.TestUpdate(scale, false);
+        if (err < err_old + p.line_search_alpha * scale * gradfdx) break;
+        scale *= p.line_search_fac;
+      }
+    }
+    s.Fixup();
+    if (((err > p.pte_slow_convergence_thresh * err_old) || niter == max_iter - 1) &&
+        close_enough) {
+      converged = true;
+      break;
+    }
+  }
+  s.Fixup();
+  st.converged = converged ? 1 : 0;
+  st.residual = err;
+  st.err = s.err;
+  return st;
+}
+
+} // namespace sgm

@@ -1,0 +1,8 @@
+// Batched persistent PTE bin kernels (2..6 materials), method PTESolverPT.
+#include "sgm_pte_kernels.inl"
+namespace sgm {
+hipError_t launch_pte_bin_pt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
+                            long n, int *counter, int block, int cpw, int grid, hipStream_t s) {
+  return kern::bin_launch<1>(nm, a, P, cells, n, counter, block, cpw, grid, s);
+}
+} // namespace sgm

@@ -1,0 +1,226 @@
+// PTE closure kernels (templates; instantiated per method in the
+// sgm_pte_*.hip translation units so the build compiles them in parallel).
+//
+//  * pte_plain_kernel: the per-call path. One lane per cell over the caller's
+//    CSR order; the material count is read per cell and dispatched to the
+//    register-resident solver instantiated for 1..8 materials (each a separate
+//    function, called once per cell).
+//  * pte_bin_kernel<NM>: the batched path. A persistent grid walks a bin of
+//    cells that all have NM materials (sorted by material signature so the
+//    EOS dispatch is wave-uniform); each wave claims `cells_per_wave` cells at
+//    a time from a device counter, so waves that drew fast-converging cells
+//    keep pulling work while slower ones finish.
+//
+// Both stage the material registry once per block into LDS (the per-material
+// parameters every EOS evaluation touches) and keep all per-cell state in
+// VGPRs. Per cell: reads rho, vfrac, temp, press per material + sie_tot,
+// vfrac_tot (+Tnorm); writes rho, vfrac, sie, temp, press + status.
+#pragma once
+#include "sgm_internal.hpp"
+
+#include <type_traits>
+
+namespace sgm {
+namespace kern {
+
+__device__ __forceinline__ void stage_registry(EosDev *lds, const EosDev *reg, int nreg) {
+  const double *s = reinterpret_cast<const double *>(reg);
+  double *d = reinterpret_cast<double *>(lds);
+  const int nd = nreg * (int)(sizeof(EosDev) / 8);
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) d[i] = s[i];
+  __syncthreads();
+}
+
+template <int NM, int METHOD>
+__device__ __forceinline__ void solve_cell(const EosDev *lds, const PteArgs &a,
+                                           const MixParamsDev &P, long c, int off) {
+  using Sys = typename std::conditional<METHOD == 0, PteRhoT<NM>, PtePT<NM>>::type;
+  Sys s;
+  Real Tin[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const int k = off + m;
+    s.eos[m] = lds + a.mat_ids[k];
+    s.rho[m] = a.rho[k];
+    s.vfrac[m] = a.vfrac[k];
+    s.press[m] = a.press[k];
+    Tin[m] = a.temp[k];
+    if (a.lambda) {
+      s.lam[m].present = true;
+      s.lam[m].lRho = a.lambda[3L * k + 0];
+      s.lam[m].lT = a.lambda[3L * k + 1];
+    } else {
+      s.lam[m].present = false;
+    }
+  }
+  s.vfrac_total = a.vfrac_tot[c];
+  s.sie_total = a.sie_tot[c];
+  s.Tnorm = a.Tnorm ? a.Tnorm[c] : 0.0;
+  s.err = 0;
+  s.P = P;
+  const CellStatus st = pte_solve<NM>(s, Tin);
+  const Real Tout = s.Tout();
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const int k = off + m;
+    a.rho[k] = s.rho[m];
+    a.vfrac[k] = s.vfrac[m];
+    a.sie[k] = s.sie[m];
+    a.temp[k] = Tout;
+    a.press[k] = s.Pout(m);
+    if (a.lambda) {
+      a.lambda[3L * k + 0] = s.lam[m].lRho;
+      a.lambda[3L * k + 1] = s.lam[m].lT;
+    }
+  }
+  if (a.conv) a.conv[c] = st.converged;
+  if (a.niter) a.niter[c] = st.niter;
+  if (a.resid) a.resid[c] = st.residual;
+  if (a.err) a.err[c] = st.err;
+}
+
+template <int NM, int METHOD>
+__device__ __attribute__((noinline)) void solve_cell_call(const EosDev *lds, const PteArgs &a,
+                                                          const MixParamsDev &P, long c,
+                                                          int off) {
+  solve_cell<NM, METHOD>(lds, a, P, c, off);
+}
+
+// validates a cell's CSR slice; returns nmat or -1
+__device__ __forceinline__ int cell_slice(const PteArgs &a, long c, int &off) {
+  const long o0 = a.offsets[c], o1 = a.offsets[c + 1];
+  if (!(o0 >= 0 && o1 >= o0 && o1 <= a.nnz)) return -1;
+  const int nm = (int)(o1 - o0);
+  if (nm < 1 || nm > MAX_MATS_PER_CELL) return -1;
+  for (int m = 0; m < nm; ++m)
+    if (a.mat_ids[o0 + m] >= a.nreg) return -1;
+  off = (int)o0;
+  return nm;
+}
+
+__device__ __forceinline__ void mark_bad(const PteArgs &a, long c) {
+  if (a.conv) a.conv[c] = 0;
+  if (a.niter) a.niter[c] = 0;
+  if (a.resid) a.resid[c] = 0;
+  if (a.err) a.err[c] = SGM_ERRB_INPUT;
+}
+
+template <int METHOD>
+__global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev P) {
+  extern __shared__ EosDev lds[];
+  stage_registry(lds, a.d_eos, a.nreg);
+  const long nthr = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < a.ncell; c += nthr) {
+    int off = 0;
+    const int nm = cell_slice(a, c, off);
+    switch (nm) {
+    case 1: solve_cell_call<1, METHOD>(lds, a, P, c, off); break;
+    case 2: solve_cell_call<2, METHOD>(lds, a, P, c, off); break;
+    case 3: solve_cell_call<3, METHOD>(lds, a, P, c, off); break;
+    case 4: solve_cell_call<4, METHOD>(lds, a, P, c, off); break;
+    case 5: solve_cell_call<5, METHOD>(lds, a, P, c, off); break;
+    case 6: solve_cell_call<6, METHOD>(lds, a, P, c, off); break;
+    case 7: solve_cell_call<7, METHOD>(lds, a, P, c, off); break;
+    case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
+    default: mark_bad(a, c); break;
+    }
+  }
+}
+
+template <int NM, int METHOD>
+__global__ void __launch_bounds__(256) pte_bin_kernel(PteArgs a, MixParamsDev P,
+                                                      const int *__restrict__ cells,
+                                                      long ncells, int *counter,
+                                                      int cells_per_wave) {
+  extern __shared__ EosDev lds[];
+  stage_registry(lds, a.d_eos, a.nreg);
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(counter, cells_per_wave);
+    base = __shfl(base, 0, 64);
+    if ((long)base >= ncells) break;
+    const long end = (long)base + cells_per_wave < ncells ? (long)base + cells_per_wave : ncells;
+    for (long j = base + lane; j < end; j += 64) {
+      const long c = cells[j];
+      int off = 0;
+      const int nm = (c >= 0 && c < a.ncell) ? cell_slice(a, c, off) : -1;
+      if (nm == NM)
+        solve_cell<NM, METHOD>(lds, a, P, c, off);
+      else if (c >= 0 && c < a.ncell)
+        mark_bad(a, c);
+    }
+  }
+}
+
+template <int METHOD>
+hipError_t plain_launch(const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
+  if (a.ncell <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+  const long blocks_needed = (a.ncell + 255) / 256;
+  const long cap = (long)device_cu_count() * 32;
+  const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
+  pte_plain_kernel<METHOD><<<g, 256, lds, s>>>(a, P);
+  return hipGetLastError();
+}
+
+template <int METHOD>
+hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                      int *counter, int block, int cpw, int grid, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+  switch (nm) {
+  case 2: pte_bin_kernel<2, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+  case 3: pte_bin_kernel<3, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+  case 4: pte_bin_kernel<4, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+  case 5: pte_bin_kernel<5, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+  case 6: pte_bin_kernel<6, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+} // namespace kern
+} // namespace sgm
+
+namespace sgm {
+namespace kern {
+// the plain solver over an explicit cell list (cells whose material count has
+// no register-resident bin kernel in the batched path: 1, 7, 8)
+template <int METHOD>
+__global__ void __launch_bounds__(256) pte_list_kernel(PteArgs a, MixParamsDev P,
+                                                       const int *__restrict__ cells, long n) {
+  extern __shared__ EosDev lds[];
+  stage_registry(lds, a.d_eos, a.nreg);
+  const long nthr = (long)gridDim.x * blockDim.x;
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += nthr) {
+    const long c = cells[j];
+    if (c < 0 || c >= a.ncell) continue;
+    int off = 0;
+    const int nm = cell_slice(a, c, off);
+    switch (nm) {
+    case 1: solve_cell_call<1, METHOD>(lds, a, P, c, off); break;
+    case 2: solve_cell_call<2, METHOD>(lds, a, P, c, off); break;
+    case 3: solve_cell_call<3, METHOD>(lds, a, P, c, off); break;
+    case 4: solve_cell_call<4, METHOD>(lds, a, P, c, off); break;
+    case 5: solve_cell_call<5, METHOD>(lds, a, P, c, off); break;
+    case 6: solve_cell_call<6, METHOD>(lds, a, P, c, off); break;
+    case 7: solve_cell_call<7, METHOD>(lds, a, P, c, off); break;
+    case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
+    default: mark_bad(a, c); break;
+    }
+  }
+}
+template <int METHOD>
+hipError_t list_launch(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+  const long blocks_needed = (n + 255) / 256;
+  const long cap = (long)device_cu_count() * 32;
+  const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
+  pte_list_kernel<METHOD><<<g, 256, lds, s>>>(a, P, cells, n);
+  return hipGetLastError();
+}
+} // namespace kern
+} // namespace sgm

@@ -1,0 +1,47 @@
+// Small helper kernels for the batched PTE plan: counter reset (a fill
+// kernel, so the reset can sit inside a captured hipGraph) and per-cell sort
+// keys (material count, then the material signature in cell order).
+#include "sgm_internal.hpp"
+
+namespace sgm {
+namespace {
+
+__global__ void zero_ints_kernel(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long ncell, long nnz,
+                                 unsigned long long *keys, int *ids) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const long o0 = offsets[c], o1 = offsets[c + 1];
+  unsigned long long k;
+  if (o0 >= 0 && o1 >= o0 && o1 <= nnz && o1 - o0 <= MAX_MATS_PER_CELL) {
+    const int nm = (int)(o1 - o0);
+    k = (unsigned long long)nm << 56;
+    for (int m = 0; m < nm && m < 7; ++m) k |= (unsigned long long)mat[o0 + m] << (48 - 8 * m);
+  } else {
+    k = 0xFFull << 56;
+  }
+  keys[c] = k;
+  ids[c] = (int)c;
+}
+
+} // namespace
+
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  zero_ints_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
+                            unsigned long long *keys, int *ids, hipStream_t s) {
+  if (ncell <= 0) return hipSuccess;
+  plan_keys_kernel<<<(int)((ncell + 255) / 256), 256, 0, s>>>(offsets, mat, ncell, nnz, keys,
+                                                                ids);
+  return hipGetLastError();
+}
+
+} // namespace sgm

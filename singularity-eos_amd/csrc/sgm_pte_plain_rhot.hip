@@ -1,0 +1,11 @@
+// Plain per-call PTE kernels, method PTESolverRhoT.
+#include "sgm_pte_kernels.inl"
+namespace sgm {
+hipError_t launch_pte_plain_rhot(const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
+  return kern::plain_launch<0>(a, P, s);
+}
+hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                             hipStream_t s) {
+  return kern::list_launch<0>(a, P, cells, n, s);
+}
+} // namespace sgm
