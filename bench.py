@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--copy-cal", action="store_true",
+                    help="end with one 1 GiB device copy (scale for FETCH/WRITE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the closure shards by cell "
@@ -137,15 +139,6 @@ def main():
         nm = b["nmat"]
         per_bin.append(dict(nmat=nm, cells=cells, ms=ms, block=b["block"],
                             cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
-    dom = max(per_bin, key=lambda r: r["ms"])
-    nm = dom["nmat"] if dom["nmat"] else 4
-    alg_bytes = dom["cells"] * bytes_per_cell(nm)
-    achieved = alg_bytes / (dom["ms"] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": f"pte_bin_kernel<{dom['nmat']},RhoT>", "cells": dom["cells"],
-                "ms": dom["ms"], "bytes_per_cell": bytes_per_cell(nm)}
-
     extras = {"bins": per_bin, "converged_frac": conv_frac, "mean_newton_iters": mean_iter}
     if not args.no_extras:
         # plain per-call path, same cells
@@ -211,7 +204,10 @@ def main():
         extras["vector_P_rho_sie_GBs"] = nq * 24 / (vms * 1e-3) / 1e9
 
     cpu = None
+    flops_per_cell = None
     if not args.no_cpu:
+        # oracle leg: CPU baseline timing, plus the algorithmic FP64 operation
+        # count per cell (the restatement built with an op-counting Real)
         import oracle as O
         O.build()
         t = table
@@ -219,20 +215,59 @@ def main():
                                        t.nT)
         olist = [O.ideal(*wl.IDEAL_A), O.ideal(*wl.IDEAL_M1), O.stiff(*wl.STIFF_B),
                  O.gruneisen(*wl.GRUN_AL), O.vinet(*wl.VINET_CU), otab.eos()]
-        ns = 1 << 16
+        specs = [(0,) + wl.IDEAL_A, (0,) + wl.IDEAL_M1, (1,) + wl.STIFF_B, (2,) + wl.GRUN_AL,
+                 (3,) + wl.VINET_CU, (4,)]
+        ns = min(1 << 16, n)
         oo = a["offsets"][: ns + 1]
         k = oo[-1]
-        r = {f: a[f][:k].copy() for f in fields}
-        threads = os.cpu_count() or 1
-        t0 = time.perf_counter()
-        O.pte_solve(0, olist, oo, a["mat_ids"][:k].astype(np.int32), r["rho"], r["vfrac"],
-                    r["sie"], r["temp"], r["press"], a["vfrac_tot"][:ns], a["sie_tot"][:ns],
-                    params.as_oracle_array(), nthreads=threads)
-        cdt = time.perf_counter() - t0
+        me = a["mat_ids"][:k].astype(np.int32)
+        threads = min(os.cpu_count() or 1, 16)
+
+        def cpu_run(ncells, nthreads):
+            kk = oo[ncells]
+            r = {f: a[f][:kk].copy() for f in fields}
+            t0 = time.perf_counter()
+            O.pte_solve(0, olist, oo[: ncells + 1], me[:kk], r["rho"], r["vfrac"], r["sie"],
+                        r["temp"], r["press"], a["vfrac_tot"][:ncells], a["sie_tot"][:ncells],
+                        params.as_oracle_array(), nthreads=nthreads)
+            return time.perf_counter() - t0
+
+        cpu_run(min(4096, ns), threads)  # warm-up
+        cdt = cpu_run(ns, threads)
+        n1 = min(8192, ns)
+        c1dt = cpu_run(n1, 1)
         cpu = {"value": ns / cdt, "unit": "cells/s", "cores": threads, "kind": "port",
                "sample": f"first {ns} cells of the same 2^20-cell uneven workload, "
-                         f"PTESolverRhoT at 1e-10 tol, OpenMP over cells ({cdt:.2f} s)"}
+                         f"PTESolverRhoT at 1e-10 tol, OpenMP over cells ({cdt:.2f} s); "
+                         f"1 thread on the first {n1} cells: {n1 / c1dt:.4g} cells/s",
+               "single_core_value": n1 / c1dt}
         log(f"[bench] cpu baseline: {cpu['value']:.4g} cells/s on {threads} threads")
+        fl = O.count_pte_flops(0, specs, oo, me, a["rho"][:k], a["vfrac"][:k], a["temp"][:k],
+                               a["press"][:k], a["vfrac_tot"][:ns], a["sie_tot"][:ns],
+                               params.as_oracle_array(), table=otab)
+        nms = nm_cell[:ns]
+        flops_per_cell = {int(m): float(fl[nms == m].mean()) for m in np.unique(nms)}
+
+    dom = max(per_bin, key=lambda r: r["ms"])
+    nm = dom["nmat"] if dom["nmat"] else 4
+    alg_bytes = dom["cells"] * bytes_per_cell(nm)
+    gbs = alg_bytes / (dom["ms"] * 1e-3) / 1e9
+    kname = f"pte_bin_kernel<{dom['nmat']},RhoT>"
+    hbm = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+           "bytes_per_cell": bytes_per_cell(nm)}
+    if flops_per_cell and nm in flops_per_cell:
+        tf = dom["cells"] * flops_per_cell[nm] / (dom["ms"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tf / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kname,
+                    "cells": dom["cells"], "ms": dom["ms"],
+                    "flops_per_cell": flops_per_cell[nm],
+                    "pipe": "FP64 on the vector ALUs; 78.6 TFLOP/s is MI355X's FP64 peak "
+                            "for vector and matrix alike (no MFMA in this kernel)"}
+        extras["roofline_hbm"] = hbm
+        extras["counted_flops_per_cell_by_nmat"] = flops_per_cell
+    else:
+        roofline = dict(hbm, cells=dom["cells"], ms=dom["ms"])
 
     out = {
         "metric": METRIC, "value": cells_per_s, "unit": "cells/s", "n_gpus": 1,
@@ -246,6 +281,11 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
     }
     print(json.dumps(out))
+    if args.copy_cal:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     plan.close()
     ctx.close()
 

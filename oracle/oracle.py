@@ -18,18 +18,55 @@ CXXFLAGS = ["-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenm
             "-shared"]
 
 
+FLOPS_LIB = os.path.join(BUILD, "liboracle_flops.so")
+
+
 def build(force=False):
-    """Compiles liboracle.so with g++ (no FMA contraction, no fast-math)."""
+    """Compiles liboracle.so with g++ (no FMA contraction, no fast-math) and
+    liboracle_flops.so, the same restatement with an operation-counting Real."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [os.path.join(HERE, s) for s in SRCS]
-    deps = srcs + [os.path.join(HERE, h) for h in HDRS]
-    if (not force and os.path.exists(LIB)
-            and os.path.getmtime(LIB) >= max(os.path.getmtime(d) for d in deps)):
-        return LIB
-    tmp = LIB + ".tmp"
-    subprocess.check_call(["g++"] + CXXFLAGS + srcs + ["-o", tmp])
-    os.replace(tmp, LIB)
+    deps = [os.path.join(HERE, s) for s in SRCS + ["flops_count.cpp"]]
+    deps += [os.path.join(HERE, h) for h in HDRS + ["sgo_real.hpp"]]
+    newest = max(os.path.getmtime(d) for d in deps)
+    for out, src, extra in ((LIB, SRCS, []), (FLOPS_LIB, ["flops_count.cpp"], ["-DSGO_COUNT_FLOPS"])):
+        if not force and os.path.exists(out) and os.path.getmtime(out) >= newest:
+            continue
+        tmp = out + ".tmp"
+        subprocess.check_call(["g++"] + CXXFLAGS + extra + [os.path.join(HERE, s) for s in src]
+                              + ["-o", tmp])
+        os.replace(tmp, out)
     return LIB
+
+
+def count_pte_flops(method, eos_specs, offsets, mat_eos, rho, vfrac, temp, press, vfrac_tot,
+                    sie_tot, params_arr, table=None):
+    """Algorithmic FP64 operations per cell (every +,-,*,/ and math call once).
+    eos_specs: list of (kind, params...) tuples; kind 4 = `table`
+    (an oracle.Table-like object with grid and field arrays)."""
+    L = C.CDLL(FLOPS_LIB)
+    f = L.sgo_count_pte_flops
+    f.restype = None
+    f.argtypes = [ci, cl, vp, vp, vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    ep = np.zeros(16 * len(eos_specs))
+    for i, spec in enumerate(eos_specs):
+        ep[16 * i:16 * i + len(spec)] = spec
+    ncell = len(offsets) - 1
+    out = np.zeros(ncell)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    mat_eos = np.ascontiguousarray(mat_eos, dtype=np.int32)
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in
+            (rho, vfrac, temp, press, vfrac_tot, sie_tot, params_arr)]
+    tg = tf = None
+    keep = []
+    if table is not None:
+        tg = np.array([table.lRhoMin, table.lRhoMax, table.nRho, table.lTMin, table.lTMax, table.nT],
+                      dtype=np.float64)
+        fl = [table.f[k] for k in Table.FIELDS] + [table.c[k] for k in Table.COLD]
+        keep = [np.ascontiguousarray(x, dtype=np.float64) for x in fl]
+        tf = (C.c_void_p * 10)(*[x.ctypes.data for x in keep])
+    f(method, ncell, p(offsets), p(mat_eos), p(ep), len(eos_specs), *[p(a) for a in arrs],
+      p(tg), tf, p(out))
+    return out
 
 
 vp, ci, cl, cd = C.c_void_p, C.c_int, C.c_long, C.c_double
@@ -145,6 +182,8 @@ class Table:
     COLD = ("PCold", "sieCold", "bModCold")
 
     def __init__(self, lRhoMin, lRhoMax, nRho, lTMin, lTMax, nT, f, c, rho_off=0.0, T_off=0.0):
+        self.lRhoMin, self.lRhoMax, self.nRho = lRhoMin, lRhoMax, nRho
+        self.lTMin, self.lTMax, self.nT = lTMin, lTMax, nT
         self.f = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in self.FIELDS}
         self.c = {k: np.ascontiguousarray(c[k], dtype=np.float64) for k in self.COLD}
         self.h = lib().sgo_table_create(lRhoMin, lRhoMax, nRho, lTMin, lTMax, nT,
@@ -226,7 +265,9 @@ def pte_solve(method, eos_list, offsets, mat_eos, rho, vfrac, sie, temp, press, 
 
 def default_params():
     """MixParams() defaults packed in the oracle order."""
-    return [3e-6, 1e-6, 1e-4, 1e-6, 1e-6, 1e-6, 1e-4, 0.99, 1e-4, 1e-3, 1e-4, 1e-4, 1e-4, 1e-3,
+    # the _sufficient values are products (1.e2 * 1.0e-6 is not 1e-4 in binary)
+    return [3e-6, 1e-6, 1e-4, 1e-6, 1e-6, 1e-6, 1e-4, 0.99, 1.e2 * 1.0e-6, 10 * 1.0e-4,
+            1.e2 * 1.0e-6, 1.e2 * 1.0e-6, 1.e2 * 1.0e-6, 10 * 1.0e-4,
             128, 1e-2, 6, 0.5, 0.95, 1e15, 300.0, 1e-16, 2, 1e-16, -1e-8, -1, 1]
 
 

@@ -164,6 +164,8 @@ void sgo_nqt(int which, const double *x, double *y, long n) {
     case 1: y[i] = fastmath::pow2(x[i]); break;
     case 2: y[i] = fastmath::log10(x[i]); break;
     case 3: y[i] = fastmath::pow10(x[i]); break;
+    case 5: y[i] = pmath::exp(x[i]); break;
+    case 6: y[i] = pmath::cbrt(x[i]); break;
     default: y[i] = to_log(x[i], 0.0); break;
     }
   }

@@ -156,7 +156,7 @@ struct Gruneisen {
       const Real minval = std::min(v1, v2), maxval = std::max(v1, v2);
       if (minval > mn && minval < mx) return minval;
       if (maxval > mn && maxval < mx) return maxval;
-      return -std::numeric_limits<Real>::infinity();
+      return Real(-std::numeric_limits<double>::infinity());
     };
     Real root = -1.;
     if (near0(s2) && near0(s3) && s1 > 1.) {
@@ -347,11 +347,11 @@ struct Vinet {
   void F_DT(Real rho, Real T, Real *out) const {
     constexpr int pref0vp = -2, pref0vip = 2, maxind = 40;
     Real sumP = 0.0, sumB = 0.0, sumE = 0.0;
-    const Real x = std::cbrt(robust::ratio(rho0, rho));
+    const Real x = pmath::cbrt(robust::ratio(rho0, rho));
     const Real x2inv = robust::ratio(1.0, x * x);
     const Real onemx = 1.0 - x;
     const Real etatimes1mx = VIP[1] * onemx;
-    const Real expetatimes1mx = std::exp(etatimes1mx);
+    const Real expetatimes1mx = pmath::exp(etatimes1mx);
     for (int ind = maxind; ind >= 2; ind--) {
       sumP = d2tod40[pref0vp + ind] + onemx * sumP;
       sumB = d2tod40[pref0vp + ind] * ind + onemx * sumB;
@@ -421,7 +421,7 @@ struct Vinet {
     F_DT(rho, t, o);
     return o[7] * o[7] * rho;
   }
-  Real MinimumDensity() const { return std::cbrt(10 * robust::EPS()) * rho0; }
+  Real MinimumDensity() const { return pmath::cbrt(10 * robust::EPS()) * rho0; }
   // eos_vinet.hpp:373-395
   void fill(Real &rho, Real &temp, Real &sie, Real &press, Real &cv_, Real &bmod,
             unsigned long output, ErrFlag &err) const {
@@ -467,7 +467,7 @@ struct Grid1D {
   }
   Real x(int i) const { return min + i * dx; }
   void weights(Real xx, int &ix, Real &w0, Real &w1) const {
-    ix = (int)std::floor((xx - min) * inv_dx);
+    ix = (int)val(std::floor((xx - min) * inv_dx));
     if (ix < 0) ix = 0;
     if (ix >= N - 1) ix = N - 2;
     const Real fl = ix * dx + min;

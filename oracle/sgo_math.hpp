@@ -17,8 +17,9 @@
 #include <cstring>
 #include <limits>
 
+#include "sgo_real.hpp"
+
 namespace sgo {
-using Real = double;
 
 // ---------------------------------------------------------------------------
 // robust:: — singularity-eos/base/robust_utils.hpp:26-68
@@ -84,6 +85,84 @@ inline double pow2(const double x) {
 inline double log10(const double x) { return 0.301029995663981195 * lg(x); }
 inline double pow10(const double x) { return pow2(3.321928094887362626 * x); }
 } // namespace fastmath
+
+// exp and cbrt built from IEEE +,-,*,/ only. The reference calls std::exp and
+// std::cbrt (eos_vinet.hpp:235-239); host and device math libraries differ in
+// the last bit, so the oracle and the device both evaluate these two with the
+// same published methods to stay bit-comparable (a CPU test bounds them
+// against the host libm):
+//  exp:  Cody-Waite reduction with fdlibm's two-part ln2, degree-13 Taylor.
+//  cbrt: exponent split x = m 2^(3q+s), quadratic seed on [1,8), five Newton steps.
+namespace pmath {
+inline double pow2i(int k) { return fastmath::as_double(static_cast<std::int64_t>(k + 1023) << 52); }
+inline double exp(double x) {
+  if (!(x == x)) return x;
+  if (x > 709.782712893384) return std::numeric_limits<double>::infinity();
+  if (x < -745.1332191019412) return 0.0;
+  const double INV_LN2 = 1.4426950408889634;
+  const double LN2_HI = 6.93147180369123816490e-01;
+  const double LN2_LO = 1.90821492927058770002e-10;
+  const double kd = std::floor(x * INV_LN2 + 0.5);
+  const double r = (x - kd * LN2_HI) - kd * LN2_LO;
+  // 1/13!, 1/12!, ..., 1/2!, 1, 1
+  static const double c[14] = {1.6059043836821613e-10, 2.08767569878681e-09,
+                               2.505210838544172e-08,  2.755731922398589e-07,
+                               2.7557319223985893e-06, 2.48015873015873e-05,
+                               0.0001984126984126984,  0.001388888888888889,
+                               0.008333333333333333,   0.041666666666666664,
+                               0.16666666666666666,    0.5, 1.0, 1.0};
+  double p = c[0];
+  for (int i = 1; i < 14; ++i) p = p * r + c[i];
+  int k = (int)kd;
+  if (k > 1023) {
+    p *= pow2i(1023);
+    k -= 1023;
+  } else if (k < -1022) {
+    p *= pow2i(k + 54);
+    k = -54;
+  }
+  return p * pow2i(k);
+}
+inline double cbrt(double x) {
+  if (x == 0.0 || !(x == x) || !(std::fabs(x) <= std::numeric_limits<double>::max())) return x;
+  double a = std::fabs(x);
+  int adj = 0;
+  if (a < std::numeric_limits<double>::min()) {
+    a *= 18014398509481984.0; // 2^54
+    adj = -18;
+  }
+  const std::int64_t bits = fastmath::as_int(a);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  const double m =
+      fastmath::as_double((bits & fastmath::mantissa_mask) | (std::int64_t(1023) << 52));
+  int q = e / 3, s = e % 3;
+  if (s < 0) {
+    s += 3;
+    q -= 1;
+  }
+  const double mm = s == 0 ? m : (s == 1 ? 2.0 * m : 4.0 * m);
+  double y = (-0.011593381976926909 * mm + 0.23742467276201396) * mm + 0.8208312163657417;
+  for (int i = 0; i < 5; ++i) y = (2.0 * y + mm / (y * y)) / 3.0;
+  y *= pow2i(q + adj);
+  return x < 0 ? -y : y;
+}
+} // namespace pmath
+
+#ifdef SGO_COUNT_FLOPS
+// counted forms: NQT lg = 1 multiply (+ integer work), pow2 = 2 multiplies and
+// a sqrt, log10/pow10 one more multiply; exp = 2 + floor + 4 (reduction) +
+// 26 (Horner) + 1 (scale) = 34; cbrt = 4 (seed) + 5 x 5 (Newton) + 1 = 30
+namespace fastmath {
+inline Real lg(Real x) { ++g_flops; return Real(lg(val(x))); }
+inline Real pow2(Real x) { g_flops += 3; return Real(pow2(val(x))); }
+inline Real log10(Real x) { g_flops += 2; return Real(log10(val(x))); }
+inline Real pow10(Real x) { g_flops += 4; return Real(pow10(val(x))); }
+} // namespace fastmath
+namespace pmath {
+inline Real exp(Real x) { g_flops += 34; return Real(exp(val(x))); }
+inline Real cbrt(Real x) { g_flops += 30; return Real(cbrt(val(x))); }
+} // namespace pmath
+#endif
 
 // spiner_common::to_log / from_log — singularity-eos/eos/eos_spiner_common.hpp:52-57
 inline Real to_log(const Real x, const Real offset) {

@@ -284,7 +284,7 @@ int sgm_set_Vinet(sgm_ctx *ctx, int m, double rho0, double T0, double B0, double
   EosDev e{};
   e.kind = KIND_VINET;
   const double p[10] = {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP[1],
-                        std::cbrt(10 * robust::EPS()) * rho0};
+                        pmath::cbrt(10 * robust::EPS()) * rho0};
   for (int i = 0; i < 10; ++i) e.p[i] = p[i];
   e.nvinet_skip = nskip;
   e.coef = v->d_coef;

@@ -512,11 +512,11 @@ SGM_NOINL EP F_DT(const Real *p, const Real *coef, int nskip, Real rho, Real T) 
   const Real *VIP = coef + 39;
   const Real VIP1 = p[8];
   Real sumP = 0.0, sumB = 0.0, sumE = 0.0;
-  const Real x = cbrt(robust::ratio(rho0, rho));
+  const Real x = pmath::cbrt(robust::ratio(rho0, rho));
   const Real x2inv = robust::ratio(1.0, x * x);
   const Real onemx = 1.0 - x;
   const Real etatimes1mx = VIP1 * onemx;
-  const Real expetatimes1mx = exp(etatimes1mx);
+  const Real expetatimes1mx = pmath::exp(etatimes1mx);
   for (int ind = 40 - nskip; ind >= 2; ind--) {
     sumP = d2tod40[ind - 2] + onemx * sumP;
     sumB = d2tod40[ind - 2] * ind + onemx * sumB;

@@ -270,6 +270,78 @@ SGM_HD bool find_root(const F &f, Real ytarget, Real xguess, Real xmin, Real xma
 }
 } // namespace root
 
+// ---------------------------------------------------------------------------
+// exp and cbrt from IEEE +,-,*,/ only, so host and device produce the same
+// bits (the Vinet model is the only caller on the hot path).
+//  exp:  Cody-Waite reduction x = k ln2 + r with fdlibm's two-part ln2
+//        (ln2_hi has 32 trailing zero bits, so k*ln2_hi is exact), |r| <= ln2/2,
+//        degree-13 Taylor polynomial (truncation < 5e-18), scaling by 2^k.
+//  cbrt: x = m 2^(3q+s), seed by a quadratic fit of cbrt on [1,8), five
+//        Newton steps y <- (2y + a/y^2)/3, scaling by 2^q.
+// Both are within a few ulp of the correctly rounded values.
+// ---------------------------------------------------------------------------
+namespace pmath {
+SGM_HD double pow2i(int k) { // 2^k for -1022 <= k <= 1023
+  return nqt::as_double(static_cast<std::int64_t>(k + 1023) << 52);
+}
+SGM_HD double exp(double x) {
+  if (!(x == x)) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  const double INV_LN2 = 1.4426950408889634;
+  const double LN2_HI = 6.93147180369123816490e-01;
+  const double LN2_LO = 1.90821492927058770002e-10;
+  const double kd = floor(x * INV_LN2 + 0.5);
+  const double r = (x - kd * LN2_HI) - kd * LN2_LO;
+  double p = 1.6059043836821613e-10;
+  p = p * r + 2.08767569878681e-09;
+  p = p * r + 2.505210838544172e-08;
+  p = p * r + 2.755731922398589e-07;
+  p = p * r + 2.7557319223985893e-06;
+  p = p * r + 2.48015873015873e-05;
+  p = p * r + 0.0001984126984126984;
+  p = p * r + 0.001388888888888889;
+  p = p * r + 0.008333333333333333;
+  p = p * r + 0.041666666666666664;
+  p = p * r + 0.16666666666666666;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  int k = (int)kd;
+  if (k > 1023) {
+    p *= pow2i(1023);
+    k -= 1023;
+  } else if (k < -1022) {
+    p *= pow2i(k + 54);
+    k = -54;
+  }
+  return p * pow2i(k);
+}
+SGM_HD double cbrt(double x) {
+  if (x == 0.0 || !(x == x) || !(fabs(x) <= 1.7976931348623157e308)) return x;
+  double a = fabs(x);
+  int adj = 0;
+  if (a < 2.2250738585072014e-308) {
+    a *= 18014398509481984.0; // 2^54
+    adj = -18;
+  }
+  const std::int64_t bits = nqt::as_int(a);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  const double m = nqt::as_double((bits & nqt::mantissa_mask) | (std::int64_t(1023) << 52));
+  int q = e / 3, s = e % 3;
+  if (s < 0) {
+    s += 3;
+    q -= 1;
+  }
+  const double mm = s == 0 ? m : (s == 1 ? 2.0 * m : 4.0 * m); // [1, 8)
+  double y = (-0.011593381976926909 * mm + 0.23742467276201396) * mm + 0.8208312163657417;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) y = (2.0 * y + mm / (y * y)) / 3.0;
+  y *= pow2i(q + adj);
+  return x < 0 ? -y : y;
+}
+} // namespace pmath
+
 // Kahan–Neumaier summation over a register array
 // (singularity-eos/closure/mixed_cell_models.hpp:287-305)
 struct NeumaierSum {

@@ -67,6 +67,19 @@ def test_nqt_log_is_close_to_log2_and_exact_at_powers_of_two(oracle):
     assert np.max(np.abs(oracle.nqt(0, r) - np.log2(r))) < 0.0099
 
 
+def test_portable_exp_cbrt_within_one_ulp_of_libm(oracle):
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-745, 709.7, 100000), rng.uniform(-1, 1, 100000)])
+    y, r = oracle.nqt(5, x), np.exp(x)
+    ok = r > 0
+    assert np.max(np.abs(y[ok] - r[ok]) / np.spacing(r[ok])) <= 1.0
+    assert oracle.nqt(5, [710.0])[0] == np.inf and oracle.nqt(5, [-746.0])[0] == 0.0
+    x = np.concatenate([10.0 ** rng.uniform(-310, 308, 100000), -(10.0 ** rng.uniform(-3, 3, 1000))])
+    y, r = oracle.nqt(6, x), np.cbrt(x)
+    assert np.max(np.abs(y - r) / np.spacing(np.abs(r))) <= 1.0
+    assert np.array_equal(oracle.nqt(6, [0.0, 8.0, -27.0, 0.125]), [0.0, 2.0, -3.0, 0.5])
+
+
 # ---------------------------------------------------------------- root finding
 def test_regula_falsi_atan(oracle):
     import ctypes as C
