@@ -211,11 +211,11 @@ def main():
         import oracle as O
         O.build()
         t = table
-        otab, _, _ = O.gruneisen_table(wl.GRUN_AL, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin, t.lTMax,
+        otab, _, _ = O.gruneisen_table(wl.GRUN_C1, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin, t.lTMax,
                                        t.nT)
         olist = [O.ideal(*wl.IDEAL_A), O.ideal(*wl.IDEAL_M1), O.stiff(*wl.STIFF_B),
-                 O.gruneisen(*wl.GRUN_AL), O.vinet(*wl.VINET_CU), otab.eos()]
-        specs = [(0,) + wl.IDEAL_A, (0,) + wl.IDEAL_M1, (1,) + wl.STIFF_B, (2,) + wl.GRUN_AL,
+                 O.gruneisen(*wl.GRUN_C1), O.vinet(*wl.VINET_CU), otab.eos()]
+        specs = [(0,) + wl.IDEAL_A, (0,) + wl.IDEAL_M1, (1,) + wl.STIFF_B, (2,) + wl.GRUN_C1,
                  (3,) + wl.VINET_CU, (4,)]
         ns = min(1 << 16, n)
         oo = a["offsets"][: ns + 1]

@@ -107,6 +107,10 @@ typedef struct sgm_table_desc {
 /* ---- context and materials --------------------------------------------- */
 /* replaces init_sg_eos (singularity-eos/eos/singularity_eos.hpp:27) */
 int sgm_init(int nmat, sgm_ctx **ctx);
+/* A context with no device (no counterpart in the reference): material setters,
+ * table queries and the seeded workload builders work on the host; every call
+ * that would launch returns SGM_E_NOGPU. For input generation and host tests. */
+int sgm_init_host(int nmat, sgm_ctx **ctx);
 /* replaces finalize_sg_eos (singularity_eos.hpp:131) */
 int sgm_finalize(sgm_ctx *ctx);
 /* replaces init_sg_IdealGas (singularity_eos.hpp:29) */
@@ -208,6 +212,73 @@ int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, do
 /* host: per-bin description, out[6*i..] = {nmat, ncells, block, cells_per_wave, grid, start} */
 int sgm_plan_describe(const sgm_plan *plan, long *out, int max_bins);
 int sgm_plan_destroy(sgm_plan *plan);
+
+
+/* ---- serving loop ---------------------------------------------------------
+ * No reference counterpart. A server keeps the context's EOS objects and
+ * tables resident on the device and serves closure and vector requests from
+ * many clients (mesh blocks). Requests name HOST arrays (pinned memory gives
+ * asynchronous copies); the client keeps them alive until the request is
+ * done. A scheduler thread admits queued requests in arrival order while
+ * their device working set fits the HBM budget, merges the admitted ones into
+ * a launch group (one merged CSR for the closures, binned by material count
+ * and signature on the device), and runs up to `nlanes` groups at once on
+ * separate streams, so requests that arrive while others run are started as
+ * soon as a lane and memory free up. Results equal the plain calls'
+ * bit for bit, alone or in any mix. */
+typedef struct {
+  size_t hbm_budget_bytes; /* device arena for request working sets (>= 64 MiB) */
+  int nlanes;              /* launch groups in flight, 1..16 */
+  int max_group_requests;  /* requests merged into one launch group, >= 1 */
+  long max_group_cells;    /* points + cells per group (a larger request runs alone) */
+  int plain;               /* 1: one request at a time through the plain per-call kernels */
+  int method;              /* closure formulation: 0 PTESolverRhoT, 1 PTESolverPT */
+} sgm_server_config;
+
+enum { SGM_REQ_VEC_P = 1, SGM_REQ_VEC_T = 2, SGM_REQ_PTE = 4 };
+
+typedef struct {
+  int kind;      /* SGM_REQ_VEC_P and/or SGM_REQ_VEC_T, or SGM_REQ_PTE */
+  int matindex;  /* vector requests: context material */
+  long n;        /* points (vector) or cells (closure) */
+  /* vector: P and/or T from (rho, sie): host rho[n], sie[n] -> P_out[n], T_out[n] */
+  const double *rho, *sie;
+  double *P_out, *T_out;
+  /* closure, like sgm_pte_rhoT: host offsets[n+1] (any starting value; the
+   * cell's entries are offsets[c]-offsets[0] .. offsets[c+1]-offsets[0] of the
+   * per-entry arrays below), mat_ids/rho_in/vfrac_in/temp_in/press_in [nnz],
+   * vfrac_tot/sie_tot [n]; results to rho_out/vfrac_out/sie_out/temp_out/
+   * press_out [nnz] and converged [n] (may be NULL) */
+  const int *offsets;
+  const uint8_t *mat_ids;
+  const double *rho_in, *vfrac_in, *temp_in, *press_in, *vfrac_tot, *sie_tot;
+  double *rho_out, *vfrac_out, *sie_out, *temp_out, *press_out;
+  int *converged;
+} sgm_request;
+
+typedef struct {
+  long submitted, completed, failed;
+  long groups;               /* launch groups run */
+  double cells;              /* points + cells completed */
+  size_t arena_peak_bytes;   /* largest arena use seen */
+  size_t arena_bytes;
+} sgm_server_stats;
+
+int sgm_server_create(sgm_ctx *ctx, const sgm_server_config *cfg, const sgm_mix_params *params,
+                      sgm_server **server);
+/* Thread-safe, non-blocking. SGM_E_BUDGET if the request alone can never fit. */
+int sgm_server_submit(sgm_server *server, const sgm_request *req, uint64_t *ticket);
+/* Blocks until the request is done: its status (SGM_OK or an error code).
+ * times (may be NULL): {submit, start, done} in seconds on the server's clock. */
+int sgm_server_wait(sgm_server *server, uint64_t ticket, double *times);
+/* Non-blocking: 1 done, 0 pending, < 0 unknown ticket. */
+int sgm_server_test(sgm_server *server, uint64_t ticket);
+/* seconds on the server's clock (steady) */
+double sgm_server_now(const sgm_server *server);
+int sgm_server_stats_get(sgm_server *server, sgm_server_stats *out);
+/* drains the queue and stops the scheduler */
+int sgm_server_destroy(sgm_server *server);
+
 
 /* ---- defaults and misc -------------------------------------------------- */
 void sgm_mix_params_default(sgm_mix_params *p);

@@ -30,7 +30,16 @@ enum TableStatus { OnTable = 0, OffBottom = 1, OffTop = 2 }; // constants.hpp:39
 struct ErrFlag {
   int code = 0;
 };
-enum ErrCode { ERR_NONE = 0, ERR_GRUN_MAXRHO = 1, ERR_GRUN_PT = 2, ERR_FILLEOS = 3 };
+// Error bits, OR'd per point or cell (the conditions the reference aborts or
+// throws on, plus the two solver bailouts of PTESolver).
+enum ErrCode {
+  ERR_NONE = 0,
+  ERR_GRUN_MAXRHO = 1,
+  ERR_GRUN_PT = 2,
+  ERR_FILLEOS = 4,
+  ERR_LINSOLVE = 8,
+  ERR_SMALLSTEP = 16
+};
 
 // ---------------------------------------------------------------------------
 // IdealGas — singularity-eos/eos/eos_ideal.hpp:38-262
@@ -58,14 +67,14 @@ struct IdealGas {
     using namespace thermalqs;
     if (output & density && output & specific_internal_energy) {
       if (output & pressure || output & temperature) {
-        err.code = ERR_FILLEOS;
+        err.code |= ERR_FILLEOS;
         return;
       }
       rhoe_PT(press, temp, rho, sie);
     }
     if (output & pressure && output & specific_internal_energy) {
       if (output & density || output & temperature) {
-        err.code = ERR_FILLEOS;
+        err.code |= ERR_FILLEOS;
         return;
       }
       sie = e_rT(rho, temp);
@@ -114,14 +123,14 @@ struct StiffGas {
     using namespace thermalqs;
     if (output & density && output & specific_internal_energy) {
       if (output & pressure || output & temperature) {
-        err.code = ERR_FILLEOS;
+        err.code |= ERR_FILLEOS;
         return;
       }
       rhoe_PT(press, temp, rho, sie);
     }
     if (output & pressure && output & specific_internal_energy) {
       if (output & density || output & temperature) {
-        err.code = ERR_FILLEOS;
+        err.code |= ERR_FILLEOS;
         return;
       }
       sie = e_rT(rho, temp);
@@ -271,7 +280,7 @@ struct Gruneisen {
     const Real slope_at_ref = dPres_drho_e(rho0, e_rT(rho0, T));
     if (slope_at_max >= 0) return rho_max;
     if (slope_at_ref < 0) {
-      err.code = ERR_GRUN_MAXRHO;
+      err.code |= ERR_GRUN_MAXRHO;
       return rho_max;
     }
     auto dPdrho_T = [this, T](Real r) { return dPres_drho_e(r, e_rT(r, T)); };
@@ -279,7 +288,7 @@ struct Gruneisen {
     const Real guess = (lo + hi) / 2.;
     Real r = hi; // defined value when the bracket search fails (the reference aborts)
     if (root::regula_falsi(dPdrho_T, 0., guess, lo, hi, 1.0e-8, 1.0e-8, r) != root::SUCCESS)
-      err.code = ERR_GRUN_MAXRHO;
+      err.code |= ERR_GRUN_MAXRHO;
     return r;
   }
   // eos_gruneisen.hpp:478-526
@@ -301,7 +310,7 @@ struct Gruneisen {
     auto PofRatT = [this, temp](Real r) { return P_rT(r, temp); };
     if (root::regula_falsi(PofRatT, p_used, guess, lo, hi, 1.0e-8, 1.0e-8, rho) !=
         root::SUCCESS)
-      err.code = ERR_GRUN_PT;
+      err.code |= ERR_GRUN_PT;
     sie = e_rT(rho, temp);
   }
   // eos_gruneisen.hpp:527-549
@@ -312,7 +321,7 @@ struct Gruneisen {
     if (temperature & input && pressure & input) {
       rhoe_PT(press, temp, rho_in, sie, err);
     } else if (density & output || specific_internal_energy & output) {
-      err.code = ERR_FILLEOS;
+      err.code |= ERR_FILLEOS;
       return;
     }
     const Real rho = std::min(rho_in, rho_max);
@@ -428,10 +437,10 @@ struct Vinet {
     using namespace thermalqs;
     const unsigned long input = ~output;
     if (density & output) {
-      err.code = ERR_FILLEOS;
+      err.code |= ERR_FILLEOS;
       return;
     } else if (temperature & output && specific_internal_energy & output) {
-      err.code = ERR_FILLEOS;
+      err.code |= ERR_FILLEOS;
       return;
     }
     if (specific_internal_energy & input) temp = T_re(rho, sie);
@@ -899,7 +908,7 @@ struct SpinerRhoT {
     TableStatus w;
     const unsigned long input = ~output;
     if (output == none) {
-      err.code = ERR_FILLEOS;
+      err.code |= ERR_FILLEOS;
       return;
     }
     if (output & density) {
@@ -908,7 +917,7 @@ struct SpinerRhoT {
         lRho = lRhoFromPlT_(press, lT, w, lambda);
         rho = rho_(lRho);
       } else {
-        err.code = ERR_FILLEOS;
+        err.code |= ERR_FILLEOS;
         return;
       }
     } else {
@@ -918,7 +927,7 @@ struct SpinerRhoT {
       if (input & density && input & specific_internal_energy) {
         lT = lTFromlRhoSie_(lRho, energy, w, lambda);
       } else {
-        err.code = ERR_FILLEOS; // (rho, P) input mode is not on the hot path
+        err.code |= ERR_FILLEOS; // (rho, P) input mode is not on the hot path
         return;
       }
       temp = T_(lT);

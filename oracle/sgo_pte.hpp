@@ -576,12 +576,14 @@ inline SolverStatus PTESolver(System &s) {
     const bool success = s.Solve();
     if (!success) {
       converged = false;
+      s.err.code |= ERR_LINSOLVE;
       break;
     }
     Real scale = std::min(1.0, s.ScaleDx());
     if (std::abs(scale) < params.pte_small_step_thresh) small_step_iters++;
     if (small_step_iters >= params.pte_small_step_tries) {
       converged = false;
+      s.err.code |= ERR_SMALLSTEP;
       break;
     }
     const Real gradfdx = -2.0 * scale * err;

@@ -64,11 +64,35 @@ class TableDesc(C.Structure):
         ("reproducible", C.c_int), ("pmin_vapor_dome", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("nlanes", C.c_int),
+                ("max_group_requests", C.c_int), ("max_group_cells", C.c_long),
+                ("plain", C.c_int), ("method", C.c_int)]
+
+
+REQ_VEC_P, REQ_VEC_T, REQ_PTE = 1, 2, 4
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("matindex", C.c_int), ("n", C.c_long)] + [
+        (k, C.c_void_p) for k in ("rho", "sie", "P_out", "T_out", "offsets", "mat_ids",
+                                  "rho_in", "vfrac_in", "temp_in", "press_in", "vfrac_tot",
+                                  "sie_tot", "rho_out", "vfrac_out", "sie_out", "temp_out",
+                                  "press_out", "converged")]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_long), ("completed", C.c_long), ("failed", C.c_long),
+                ("groups", C.c_long), ("cells", C.c_double), ("arena_peak_bytes", C.c_size_t),
+                ("arena_bytes", C.c_size_t)]
+
+
 _lib = None
 vp, ci, cl, cd = C.c_void_p, C.c_int, C.c_long, C.c_double
 
 _SIGS = {
     "sgm_init": (ci, [ci, C.POINTER(vp)]),
+    "sgm_init_host": (ci, [ci, C.POINTER(vp)]),
     "sgm_finalize": (ci, [vp]),
     "sgm_set_IdealGas": (ci, [vp, ci, cd, cd]),
     "sgm_set_StiffGas": (ci, [vp, ci, cd, cd, cd, cd]),
@@ -100,6 +124,14 @@ _SIGS = {
     "sgm_version": (C.c_char_p, []),
     "sgm_build_gruneisen_table": (ci, [cd] * 10 + [cd, cd, ci, cd, cd, ci] + [vp] * 10),
     "sgm_workload_pte2": (ci, [C.c_uint64, cl, vp, vp, vp, vp, vp, vp]),
+    "sgm_server_create": (ci, [vp, C.POINTER(ServerConfig), C.POINTER(MixParams),
+                               C.POINTER(vp)]),
+    "sgm_server_submit": (ci, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "sgm_server_wait": (ci, [vp, C.c_uint64, C.POINTER(C.c_double)]),
+    "sgm_server_test": (ci, [vp, C.c_uint64]),
+    "sgm_server_now": (cd, [vp]),
+    "sgm_server_stats_get": (ci, [vp, C.POINTER(ServerStats)]),
+    "sgm_server_destroy": (ci, [vp]),
     "sgm_workload_uneven": (cl, [vp, C.c_uint64, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
 }
 
@@ -151,12 +183,18 @@ def check(rc, what="sgm call"):
 
 
 class Context:
-    """A material registry on the current GPU (sgm_ctx)."""
+    """A material registry on the current GPU (sgm_ctx). host_only=True makes
+    a registry with no device (setters, table queries and the workload
+    builders only; launching calls fail with SGM_E_NOGPU)."""
 
-    def __init__(self, nmat):
+    def __init__(self, nmat, host_only=False):
         self.h = vp()
-        check(lib().sgm_init(nmat, C.byref(self.h)), "sgm_init")
+        if host_only:
+            check(lib().sgm_init_host(nmat, C.byref(self.h)), "sgm_init_host")
+        else:
+            check(lib().sgm_init(nmat, C.byref(self.h)), "sgm_init")
         self.nmat = nmat
+        self.host_only = host_only
         self._keep = []
 
     def close(self):
@@ -274,6 +312,92 @@ class Plan:
         if self.h:
             lib().sgm_plan_destroy(self.h)
             self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over a context (sgm_server). Request arrays are HOST
+    numpy arrays or CPU tensors (pinned memory copies asynchronously); they
+    must stay alive until the request is done (the Server keeps a reference)."""
+
+    def __init__(self, ctx, budget_bytes=8 << 30, nlanes=4, max_group_requests=64,
+                 max_group_cells=1 << 21, plain=False, method=0, params=None):
+        self.h = vp()
+        self.ctx = ctx
+        cfg = ServerConfig(budget_bytes, nlanes, max_group_requests, max_group_cells,
+                           1 if plain else 0, method)
+        check(lib().sgm_server_create(ctx.h, C.byref(cfg),
+                                      C.byref(params) if params is not None else None,
+                                      C.byref(self.h)), "server_create")
+        self._live = {}
+
+    @staticmethod
+    def _p(x):
+        if x is None:
+            return None
+        if hasattr(x, "data_ptr"):
+            return x.data_ptr()
+        return x.ctypes.data
+
+    def submit_vec(self, m, rho, sie, P_out=None, T_out=None):
+        kind = (REQ_VEC_P if P_out is not None else 0) | (REQ_VEC_T if T_out is not None else 0)
+        r = Request()
+        r.kind, r.matindex, r.n = kind, m, len(rho)
+        r.rho, r.sie = self._p(rho), self._p(sie)
+        r.P_out, r.T_out = self._p(P_out), self._p(T_out)
+        return self._submit(r, (rho, sie, P_out, T_out))
+
+    def submit_pte(self, offsets, mat_ids, rho, vfrac, temp, press, vfrac_tot, sie_tot, out,
+                   converged=None):
+        """out: dict with rho, vfrac, sie, temp, press host arrays [nnz]."""
+        r = Request()
+        r.kind, r.n = REQ_PTE, len(offsets) - 1
+        r.offsets, r.mat_ids = self._p(offsets), self._p(mat_ids)
+        r.rho_in, r.vfrac_in, r.temp_in, r.press_in = (self._p(x) for x in (rho, vfrac, temp,
+                                                                            press))
+        r.vfrac_tot, r.sie_tot = self._p(vfrac_tot), self._p(sie_tot)
+        r.rho_out, r.vfrac_out, r.sie_out, r.temp_out, r.press_out = (
+            self._p(out[k]) for k in ("rho", "vfrac", "sie", "temp", "press"))
+        r.converged = self._p(converged)
+        return self._submit(r, (offsets, mat_ids, rho, vfrac, temp, press, vfrac_tot, sie_tot,
+                                out, converged))
+
+    def _submit(self, r, keep):
+        t = C.c_uint64()
+        rc = lib().sgm_server_submit(self.h, C.byref(r), C.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"server_submit failed with code {rc}")
+        self._live[t.value] = keep
+        return t.value
+
+    def wait(self, ticket):
+        """-> (status, (t_submit, t_start, t_done))"""
+        tm = (C.c_double * 3)()
+        rc = lib().sgm_server_wait(self.h, ticket, tm)
+        self._live.pop(ticket, None)
+        return rc, (tm[0], tm[1], tm[2])
+
+    def done(self, ticket):
+        return lib().sgm_server_test(self.h, ticket) == 1
+
+    def now(self):
+        return lib().sgm_server_now(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().sgm_server_stats_get(self.h, C.byref(st)), "server_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().sgm_server_destroy(self.h)
+            self.h = vp()
+            self._live.clear()
 
     def __del__(self):
         try:

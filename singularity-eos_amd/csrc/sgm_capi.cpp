@@ -16,6 +16,7 @@ hipError_t upload_table(TableHost &T);
 namespace {
 
 int sync_registry(sgm_ctx *ctx) {
+  if (ctx->host_only) return SGM_OK;
   if (!ctx->dirty) return SGM_OK;
   if (hipMemcpy(ctx->d_eos, ctx->host_eos.data(), sizeof(EosDev) * ctx->nmat,
                 hipMemcpyHostToDevice) != hipSuccess)
@@ -28,6 +29,13 @@ int check_mat(const sgm_ctx *ctx, int m) {
   if (!ctx) return SGM_E_ARG;
   if (m < 0 || m >= ctx->nmat || !ctx->is_set[m]) return SGM_E_MAT;
   return SGM_OK;
+}
+
+// every entry point that launches work starts here
+int check_launch(const sgm_ctx *ctx, int m) {
+  const int r = check_mat(ctx, m);
+  if (r != SGM_OK) return r;
+  return ctx->host_only ? SGM_E_NOGPU : SGM_OK;
 }
 
 MixParamsDev to_dev(const sgm_mix_params *p) {
@@ -202,8 +210,27 @@ int sgm_init(int nmat, sgm_ctx **out) {
   return SGM_OK;
 }
 
+int sgm_init_host(int nmat, sgm_ctx **out) {
+  if (!out || nmat < 1 || nmat > MAX_REGISTRY) return SGM_E_ARG;
+  auto *ctx = new sgm_ctx();
+  ctx->host_only = true;
+  ctx->device = -1;
+  ctx->nmat = nmat;
+  ctx->host_eos.assign(nmat, EosDev{});
+  ctx->host_eval.assign(nmat, EosDev{});
+  ctx->is_set.assign(nmat, 0);
+  ctx->tables.resize(nmat);
+  ctx->vinets.resize(nmat);
+  *out = ctx;
+  return SGM_OK;
+}
+
 int sgm_finalize(sgm_ctx *ctx) {
   if (!ctx) return SGM_E_ARG;
+  if (ctx->host_only) {
+    delete ctx;
+    return SGM_OK;
+  }
   hipDeviceSynchronize();
   for (auto &t : ctx->tables)
     if (t)
@@ -277,10 +304,12 @@ int sgm_set_Vinet(sgm_ctx *ctx, int m, double rho0, double T0, double B0, double
     else
       break;
   }
-  if (hipMalloc(&v->d_coef, v->coef.size() * sizeof(Real)) != hipSuccess) return SGM_E_HIP;
-  if (hipMemcpy(v->d_coef, v->coef.data(), v->coef.size() * sizeof(Real),
-                hipMemcpyHostToDevice) != hipSuccess)
-    return SGM_E_HIP;
+  if (!ctx->host_only) {
+    if (hipMalloc(&v->d_coef, v->coef.size() * sizeof(Real)) != hipSuccess) return SGM_E_HIP;
+    if (hipMemcpy(v->d_coef, v->coef.data(), v->coef.size() * sizeof(Real),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return SGM_E_HIP;
+  }
   EosDev e{};
   e.kind = KIND_VINET;
   const double p[10] = {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP[1],
@@ -307,7 +336,7 @@ int sgm_set_TableRhoT(sgm_ctx *ctx, int m, const sgm_table_desc *desc) {
   if (m < 0 || m >= ctx->nmat) return SGM_E_MAT;
   auto T = std::make_unique<TableHost>();
   if (!build_table_host(*desc, *T)) return SGM_E_ARG;
-  if (upload_table(*T) != hipSuccess) return SGM_E_HIP;
+  if (!ctx->host_only && upload_table(*T) != hipSuccess) return SGM_E_HIP;
   EosDev e{};
   e.kind = KIND_TABLE;
   e.tab = T->d_tab;
@@ -355,7 +384,7 @@ long sgm_table_query(const sgm_ctx *ctx, int m, int which, double *out) {
 
 int sgm_vec_eval(sgm_ctx *ctx, int m, int func, const double *in1, const double *in2,
                  double *out, long n, int lstride, double *lambda, sgm_stream_t stream) {
-  const int r = check_mat(ctx, m);
+  const int r = check_launch(ctx, m);
   if (r != SGM_OK) return r;
   if (n < 0 || func < 0 || func > SGM_F_MIN_SIE_RHO) return SGM_E_ARG;
   if (n > 0 && (!in1 || !out || (func != SGM_F_MIN_SIE_RHO && !in2))) return SGM_E_ARG;
@@ -379,7 +408,7 @@ int sgm_vec_T_from_rho_sie(sgm_ctx *ctx, int m, const double *rho, const double 
 int sgm_vec_fill_eos(sgm_ctx *ctx, int m, double *rho, double *temp, double *sie, double *press,
                      double *cv, double *bmod, long n, unsigned long output, int lstride,
                      double *lambda, int *errs, sgm_stream_t stream) {
-  const int r = check_mat(ctx, m);
+  const int r = check_launch(ctx, m);
   if (r != SGM_OK) return r;
   if (n < 0 || (lambda && lstride < 2)) return SGM_E_ARG;
   // density is always read or written; every requested output needs its
@@ -402,7 +431,7 @@ int sgm_vec_fill_eos(sgm_ctx *ctx, int m, double *rho, double *temp, double *sie
 int sgm_vec_rhoe_from_PT(sgm_ctx *ctx, int m, const double *P, const double *T, double *rho,
                          double *sie, long n, int lstride, double *lambda, int *errs,
                          sgm_stream_t stream) {
-  const int r = check_mat(ctx, m);
+  const int r = check_launch(ctx, m);
   if (r != SGM_OK) return r;
   if (n < 0 || (n > 0 && (!P || !T || !rho || !sie)) || (lambda && lstride < 2)) return SGM_E_ARG;
   if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
@@ -418,6 +447,7 @@ static int pte_common(int method, sgm_ctx *ctx, long ncell, long nnz, const int 
                       const double *sie_tot, const double *Tnorm, double *lambda,
                       const sgm_mix_params *params, sgm_pte_status st, sgm_stream_t stream) {
   if (!ctx || ncell < 0 || nnz < 0) return SGM_E_ARG;
+  if (ctx->host_only) return SGM_E_NOGPU;
   if (ncell == 0) return SGM_OK;
   if (!offsets || !mat_ids || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot ||
       !sie_tot)

@@ -36,6 +36,7 @@ struct VinetHost {
 
 struct sgm_ctx {
   int device = 0;
+  bool host_only = false; // sgm_init_host: no device copies, launches refused
   int nmat = 0;
   std::vector<sgm::EosDev> host_eos; // device-pointer view (what kernels get)
   std::vector<sgm::EosDev> host_eval; // host-pointer view (for host-side evaluation)
@@ -83,6 +84,18 @@ hipError_t launch_pte_bin_rhot(int nm, const PteArgs &a, const MixParamsDev &P, 
                                long n, int *counter, int block, int cpw, int grid, hipStream_t s);
 hipError_t launch_pte_bin_pt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
                              long n, int *counter, int block, int cpw, int grid, hipStream_t s);
+// the same kernels with the bin bounds read on the device from range = {start, count}
+// (cells into `cells`); nmax bounds count for the list kernel's grid
+hipError_t launch_pte_bin_rhot_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
+                                      const int *cells, const long *range, int *counter,
+                                      int block, int cpw, int grid, hipStream_t s);
+hipError_t launch_pte_bin_pt_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
+                                    const int *cells, const long *range, int *counter, int block,
+                                    int cpw, int grid, hipStream_t s);
+hipError_t launch_pte_list_rhot_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
+                                       const long *range, long nmax, hipStream_t s);
+hipError_t launch_pte_list_pt_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
+                                     const long *range, long nmax, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s);

@@ -17,7 +17,9 @@ namespace {
 struct Bin {
   int nm;          // 2..6 register-resident kernels, 0 = generic list kernel
   long start, n;   // range in the sorted cell list
-  int block = 256, cpw = 64, grid = 0;
+  int block = 256;
+  int cpw = 64;    // cells a wave claims per atomic on the bin counter (>= 64)
+  int grid = 0;
 };
 
 struct GraphKey {
@@ -37,6 +39,10 @@ struct sgm_plan {
   int *d_counters = nullptr;
   std::vector<Bin> bins;
   hipStream_t cap_stream = nullptr;
+  // capture-time fork streams: each bin's kernel is its own graph branch, so
+  // the bins' tails (the slowest cells) overlap instead of adding up
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> ev;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool have_key = false;
@@ -56,23 +62,39 @@ int occupancy_grid(int method, int nm, int block) {
   return g;
 }
 
-hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
+hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixParamsDev &P,
+                          hipStream_t s) {
+  const Bin &b = p->bins[i];
+  const int *cells = p->d_sorted + b.start;
+  if (b.nm >= 2 && b.nm <= 6)
+    return p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + i,
+                                                b.block, b.cpw, b.grid, s)
+                          : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
+                                              b.cpw, b.grid, s);
+  return p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
+                        : launch_pte_list_pt(a, P, cells, b.n, s);
+}
+
+// Records the counter reset and every bin on stream s. With fork streams
+// (graph capture) bin i runs on aux[i] between a fork and a join event.
+hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hipStream_t s,
+                       bool fork) {
   hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
   if (e != hipSuccess) return e;
-  for (size_t i = 0; i < p->bins.size(); ++i) {
-    const Bin &b = p->bins[i];
-    const int *cells = p->d_sorted + b.start;
-    if (b.nm >= 2 && b.nm <= 6) {
-      e = p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
-                                               b.cpw, b.grid, s)
-                         : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
-                                             b.cpw, b.grid, s);
-    } else {
-      e = p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
-                         : launch_pte_list_pt(a, P, cells, b.n, s);
-    }
-    if (e != hipSuccess) return e;
+  const size_t nb = p->bins.size();
+  if (!fork || nb < 2 || p->aux.size() < nb || p->ev.size() < nb + 1) {
+    for (size_t i = 0; i < nb; ++i)
+      if ((e = launch_one_bin(p, i, a, P, s)) != hipSuccess) return e;
+    return hipSuccess;
   }
+  if ((e = hipEventRecord(p->ev[0], s)) != hipSuccess) return e;
+  for (size_t i = 0; i < nb; ++i) {
+    if ((e = hipStreamWaitEvent(p->aux[i], p->ev[0], 0)) != hipSuccess) return e;
+    if ((e = launch_one_bin(p, i, a, P, p->aux[i])) != hipSuccess) return e;
+    if ((e = hipEventRecord(p->ev[i + 1], p->aux[i])) != hipSuccess) return e;
+  }
+  for (size_t i = 0; i < nb; ++i)
+    if ((e = hipStreamWaitEvent(s, p->ev[i + 1], 0)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -123,6 +145,7 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
       !offsets || !mat_ids || (method != 0 && method != 1))
     return SGM_E_ARG;
   *out = nullptr;
+  if (ctx->host_only) return SGM_E_NOGPU;
   hipStream_t s = (hipStream_t)stream;
   auto *p = new sgm_plan();
   p->ctx = ctx;
@@ -192,6 +215,22 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
       hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess) {
     sgm_plan_destroy(p);
     return SGM_E_HIP;
+  }
+  for (size_t i = 0; i < p->bins.size(); ++i) {
+    hipStream_t q = nullptr;
+    if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+      sgm_plan_destroy(p);
+      return SGM_E_HIP;
+    }
+    p->aux.push_back(q);
+  }
+  for (size_t i = 0; i <= p->bins.size(); ++i) {
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      sgm_plan_destroy(p);
+      return SGM_E_HIP;
+    }
+    p->ev.push_back(ev);
   }
   *out = p;
   return SGM_OK;
@@ -323,7 +362,7 @@ int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, doubl
                           status);
     if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return SGM_E_HIP;
-    const hipError_t le = launch_bins(p, a, P, p->cap_stream);
+    const hipError_t le = launch_bins(p, a, P, p->cap_stream, true);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(p->cap_stream, &g);
     if (le != hipSuccess || ce != hipSuccess || !g) {
@@ -354,18 +393,9 @@ int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, doubl
   const MixParamsDev P = params_dev(params);
   PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
                         status);
-  const Bin &b = p->bins[bin];
   hipError_t e = launch_zero_ints(p->d_counters + bin, 1, s);
   if (e != hipSuccess) return SGM_E_HIP;
-  const int *cells = p->d_sorted + b.start;
-  if (b.nm >= 2 && b.nm <= 6)
-    e = p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + bin,
-                                             b.block, b.cpw, b.grid, s)
-                       : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + bin, b.block,
-                                           b.cpw, b.grid, s);
-  else
-    e = p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
-                       : launch_pte_list_pt(a, P, cells, b.n, s);
+  e = launch_one_bin(p, (size_t)bin, a, P, s);
   return e == hipSuccess ? SGM_OK : SGM_E_HIP;
 }
 
@@ -379,6 +409,8 @@ int sgm_plan_destroy(sgm_plan *p) {
   if (p->d_sorted) hipFree(p->d_sorted);
   if (p->d_counters) hipFree(p->d_counters);
   if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+  for (hipStream_t q : p->aux) hipStreamDestroy(q);
+  for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
   delete p;
   return SGM_OK;
 }

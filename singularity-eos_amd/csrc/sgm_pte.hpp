@@ -724,60 +724,92 @@ struct PtePT : PteBase<NM, 2> {
 // ---------------------------------------------------------------------------
 // PTESolver driver (:2188-2291)
 // ---------------------------------------------------------------------------
+// The driver is split into begin / one iteration / end so a lane can retire
+// its cell and start the next one between iterations (the batched kernels);
+// pte_solve chains them for one cell. The iteration body is the reference's
+// loop body, in its order.
+struct DriverState {
+  Real err;
+  int niter, small_step_iters, max_iter;
+  bool converged, close_enough;
+};
+
 template <int NM, class S>
-SGM_HD CellStatus pte_solve(S &s, const Real (&Tin)[NM]) {
-  const MixParamsDev &p = s.P;
-  CellStatus st;
+SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &st) {
   st.converged = 0;
   st.niter = 0;
-  Real err = s.Init(Tin);
-  bool converged = false, close_enough = false;
-  const int max_iter = NM * p.pte_max_iter_per_mat;
-  int small_step_iters = 0;
-  int niter;
-  for (niter = 0; niter < max_iter; ++niter) {
-    st.niter = niter;
-    s.CheckPTE(converged, close_enough);
-    if (converged) break;
-    s.Jacobian();
-    if (!s.Solve()) {
-      converged = false;
-      s.err |= ERRB_LINSOLVE;
-      break;
-    }
-    Real scale = smin(1.0, s.ScaleDx());
-    if (fabs(scale) < p.pte_small_step_thresh) small_step_iters++;
-    if (small_step_iters >= p.pte_small_step_tries) {
-      converged = false;
-      s.err |= ERRB_SMALLSTEP;
-      break;
-    }
-    const Real gradfdx = -2.0 * scale * err;
-    scale = 1.0;
-    const Real err_old = err;
-    err = s.TestUpdate(scale, true);
-    if (err > err_old + p.line_search_alpha * gradfdx) {
-      scale = 0.5;
-      const Real err_mid = s.TestUpdate(scale, false);
-      if (err_mid < err && err_mid < err_old)
-        scale = 0.75 + 0.5 * robust::ratio(err_mid - err, err - 2.0 * err_mid + err_old);
-      for (int li = 0; li < p.line_search_max_iter; li++) {
-        err = s.TestUpdate(scale, false);
-        if (err < err_old + p.line_search_alpha * scale * gradfdx) break;
-        scale *= p.line_search_fac;
-      }
-    }
-    s.Fixup();
-    if (((err > p.pte_slow_convergence_thresh * err_old) || niter == max_iter - 1) &&
-        close_enough) {
-      converged = true;
-      break;
+  d.err = s.Init(Tin);
+  d.converged = false;
+  d.close_enough = false;
+  d.max_iter = NM * s.P.pte_max_iter_per_mat;
+  d.small_step_iters = 0;
+  d.niter = 0;
+}
+
+// one pass of `for (niter = 0; niter < max_iter; ++niter)`; true once the
+// loop has ended (break or exhausted)
+template <int NM, class S>
+SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
+  const MixParamsDev &p = s.P;
+  if (d.niter >= d.max_iter) return true;
+  st.niter = d.niter;
+  s.CheckPTE(d.converged, d.close_enough);
+  if (d.converged) return true;
+  s.Jacobian();
+  if (!s.Solve()) {
+    d.converged = false;
+    s.err |= ERRB_LINSOLVE;
+    return true;
+  }
+  Real scale = smin(1.0, s.ScaleDx());
+  if (fabs(scale) < p.pte_small_step_thresh) d.small_step_iters++;
+  if (d.small_step_iters >= p.pte_small_step_tries) {
+    d.converged = false;
+    s.err |= ERRB_SMALLSTEP;
+    return true;
+  }
+  const Real gradfdx = -2.0 * scale * d.err;
+  scale = 1.0;
+  const Real err_old = d.err;
+  Real err = s.TestUpdate(scale, true);
+  if (err > err_old + p.line_search_alpha * gradfdx) {
+    scale = 0.5;
+    const Real err_mid = s.TestUpdate(scale, false);
+    if (err_mid < err && err_mid < err_old)
+      scale = 0.75 + 0.5 * robust::ratio(err_mid - err, err - 2.0 * err_mid + err_old);
+    for (int li = 0; li < p.line_search_max_iter; li++) {
+      err = s.TestUpdate(scale, false);
+      if (err < err_old + p.line_search_alpha * scale * gradfdx) break;
+      scale *= p.line_search_fac;
     }
   }
+  d.err = err;
   s.Fixup();
-  st.converged = converged ? 1 : 0;
-  st.residual = err;
+  if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&
+      d.close_enough) {
+    d.converged = true;
+    return true;
+  }
+  ++d.niter;
+  return d.niter >= d.max_iter;
+}
+
+template <int NM, class S>
+SGM_HD void pte_end(S &s, const DriverState &d, CellStatus &st) {
+  s.Fixup();
+  st.converged = d.converged ? 1 : 0;
+  st.residual = d.err;
   st.err = s.err;
+}
+
+template <int NM, class S>
+SGM_HD CellStatus pte_solve(S &s, const Real (&Tin)[NM]) {
+  CellStatus st;
+  DriverState d;
+  pte_begin<NM>(s, Tin, d, st);
+  while (!pte_iterate<NM>(s, d, st)) {
+  }
+  pte_end<NM>(s, d, st);
   return st;
 }
 

@@ -5,4 +5,9 @@ hipError_t launch_pte_bin_rhot(int nm, const PteArgs &a, const MixParamsDev &P, 
                             long n, int *counter, int block, int cpw, int grid, hipStream_t s) {
   return kern::bin_launch<0>(nm, a, P, cells, n, counter, block, cpw, grid, s);
 }
+hipError_t launch_pte_bin_rhot_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
+                                    const int *cells, const long *range, int *counter, int block,
+                                    int cpw, int grid, hipStream_t s) {
+  return kern::bin_launch<0>(nm, a, P, cells, 0, counter, block, cpw, grid, s, range);
+}
 } // namespace sgm

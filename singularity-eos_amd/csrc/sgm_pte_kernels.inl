@@ -7,9 +7,9 @@
 //    function, called once per cell).
 //  * pte_bin_kernel<NM>: the batched path. A persistent grid walks a bin of
 //    cells that all have NM materials (sorted by material signature so the
-//    EOS dispatch is wave-uniform); each wave claims `cells_per_wave` cells at
-//    a time from a device counter, so waves that drew fast-converging cells
-//    keep pulling work while slower ones finish.
+//    EOS dispatch is wave-uniform); lanes retire finished cells and take new
+//    ones between Newton iterations, from a per-wave reserve refilled
+//    `chunk` (>= 64) cells at a time from a device counter.
 //
 // Both stage the material registry once per block into LDS (the per-material
 // parameters every EOS evaluation touches) and keep all per-cell state in
@@ -32,11 +32,12 @@ __device__ __forceinline__ void stage_registry(EosDev *lds, const EosDev *reg, i
 }
 
 template <int NM, int METHOD>
-__device__ __forceinline__ void solve_cell(const EosDev *lds, const PteArgs &a,
-                                           const MixParamsDev &P, long c, int off) {
-  using Sys = typename std::conditional<METHOD == 0, PteRhoT<NM>, PtePT<NM>>::type;
-  Sys s;
-  Real Tin[NM];
+using SysOf = typename std::conditional<METHOD == 0, PteRhoT<NM>, PtePT<NM>>::type;
+
+template <int NM, class Sys>
+__device__ __forceinline__ void load_cell(Sys &s, Real (&Tin)[NM], const EosDev *lds,
+                                          const PteArgs &a, const MixParamsDev &P, long c,
+                                          int off) {
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     const int k = off + m;
@@ -58,7 +59,11 @@ __device__ __forceinline__ void solve_cell(const EosDev *lds, const PteArgs &a,
   s.Tnorm = a.Tnorm ? a.Tnorm[c] : 0.0;
   s.err = 0;
   s.P = P;
-  const CellStatus st = pte_solve<NM>(s, Tin);
+}
+
+template <int NM, class Sys>
+__device__ __forceinline__ void store_cell(const Sys &s, const CellStatus &st, const PteArgs &a,
+                                           long c, int off) {
   const Real Tout = s.Tout();
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
@@ -77,6 +82,16 @@ __device__ __forceinline__ void solve_cell(const EosDev *lds, const PteArgs &a,
   if (a.niter) a.niter[c] = st.niter;
   if (a.resid) a.resid[c] = st.residual;
   if (a.err) a.err[c] = st.err;
+}
+
+template <int NM, int METHOD>
+__device__ __forceinline__ void solve_cell(const EosDev *lds, const PteArgs &a,
+                                           const MixParamsDev &P, long c, int off) {
+  SysOf<NM, METHOD> s;
+  Real Tin[NM];
+  load_cell<NM>(s, Tin, lds, a, P, c, off);
+  const CellStatus st = pte_solve<NM>(s, Tin);
+  store_cell<NM>(s, st, a, c, off);
 }
 
 template <int NM, int METHOD>
@@ -127,28 +142,82 @@ __global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev 
   }
 }
 
+// Batched bin kernel with lane refill: every lane owns one cell and runs one
+// Newton iteration of it per trip of the loop; a lane whose cell has finished
+// stores it and takes the next cell of the bin at once, so a wave never idles
+// behind its slowest cell. Cells come from a per-wave reserve that the wave
+// refills `chunk` cells at a time with one atomic on the bin counter; the
+// sorted bin order keeps a wave's cells on the same material signature.
 template <int NM, int METHOD>
 __global__ void __launch_bounds__(256) pte_bin_kernel(PteArgs a, MixParamsDev P,
                                                       const int *__restrict__ cells,
-                                                      long ncells, int *counter,
-                                                      int cells_per_wave) {
+                                                      long ncells, int *counter, int chunk,
+                                                      const long *range) {
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
+  if (range) {  // bin bounds computed on the device: {start, count} into cells
+    cells += range[0];
+    ncells = range[1];
+  }
   const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (chunk < 64) chunk = 64;  // one claim must cover a whole wave
+  SysOf<NM, METHOD> s;
+  DriverState d;
+  CellStatus st;
+  long c = -1;
+  int off = 0;
+  bool active = false;  // this lane holds a cell that is still iterating
+  bool done = false;    // the bin is exhausted for this wave
+  long rbase = 0, rend = 0;  // wave-uniform reserve [rbase, rend) of bin positions
   for (;;) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(counter, cells_per_wave);
-    base = __shfl(base, 0, 64);
-    if ((long)base >= ncells) break;
-    const long end = (long)base + cells_per_wave < ncells ? (long)base + cells_per_wave : ncells;
-    for (long j = base + lane; j < end; j += 64) {
-      const long c = cells[j];
-      int off = 0;
-      const int nm = (c >= 0 && c < a.ncell) ? cell_slice(a, c, off) : -1;
-      if (nm == NM)
-        solve_cell<NM, METHOD>(lds, a, P, c, off);
-      else if (c >= 0 && c < a.ncell)
-        mark_bad(a, c);
+    // hand out cells to idle lanes
+    const unsigned long long need = __ballot(!active && !done);
+    if (need) {
+      const long cnt = __popcll(need);
+      const long rank = __popcll(need & below);
+      long j = -1;
+      if (rank < rend - rbase) {
+        j = rbase + rank;
+      }
+      if (rend - rbase < cnt) {
+        int b = 0;
+        if (lane == 0) b = atomicAdd(counter, chunk);
+        b = __shfl(b, 0, 64);
+        const long avail = rend - rbase;
+        if (rank >= avail) j = (long)b + (rank - avail);
+        rbase = (long)b + (cnt - avail);
+        rend = (long)b + chunk;
+      } else {
+        rbase += cnt;
+      }
+      if (rend > ncells) rend = ncells;
+      if (rbase > rend) rbase = rend;
+      if (!active && !done) {
+        if (j >= 0 && j < ncells) {
+          c = cells[j];
+          const int nm = (c >= 0 && c < a.ncell) ? cell_slice(a, c, off) : -1;
+          if (nm == NM) {
+            Real Tin[NM];
+            load_cell<NM>(s, Tin, lds, a, P, c, off);
+            pte_begin<NM>(s, Tin, d, st);
+            active = true;
+          } else if (c >= 0 && c < a.ncell) {
+            mark_bad(a, c);
+          }
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (__all(done)) break;
+      continue;
+    }
+    if (active && pte_iterate<NM>(s, d, st)) {
+      pte_end<NM>(s, d, st);
+      store_cell<NM>(s, st, a, c, off);
+      active = false;
     }
   }
 }
@@ -166,17 +235,23 @@ hipError_t plain_launch(const PteArgs &a, const MixParamsDev &P, hipStream_t s) 
 
 template <int METHOD>
 hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                      int *counter, int block, int cpw, int grid, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
+                      int *counter, int block, int cpw, int grid, hipStream_t s,
+                      const long *range = nullptr) {
+  if (n <= 0 && !range) return hipSuccess;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+#define SGM_BIN(K)                                                                             \
+  case K:                                                                                      \
+    pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range);   \
+    break;
   switch (nm) {
-  case 2: pte_bin_kernel<2, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
-  case 3: pte_bin_kernel<3, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
-  case 4: pte_bin_kernel<4, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
-  case 5: pte_bin_kernel<5, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
-  case 6: pte_bin_kernel<6, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw); break;
+    SGM_BIN(2)
+    SGM_BIN(3)
+    SGM_BIN(4)
+    SGM_BIN(5)
+    SGM_BIN(6)
   default: return hipErrorInvalidValue;
   }
+#undef SGM_BIN
   return hipGetLastError();
 }
 
@@ -189,9 +264,14 @@ namespace kern {
 // no register-resident bin kernel in the batched path: 1, 7, 8)
 template <int METHOD>
 __global__ void __launch_bounds__(256) pte_list_kernel(PteArgs a, MixParamsDev P,
-                                                       const int *__restrict__ cells, long n) {
+                                                       const int *__restrict__ cells, long n,
+                                                       const long *range) {
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
+  if (range) {
+    cells += range[0];
+    n = range[1];
+  }
   const long nthr = (long)gridDim.x * blockDim.x;
   for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += nthr) {
     const long c = cells[j];
@@ -211,15 +291,16 @@ __global__ void __launch_bounds__(256) pte_list_kernel(PteArgs a, MixParamsDev P
     }
   }
 }
+// range (device {start, count}): n is then the host's upper bound on count
 template <int METHOD>
 hipError_t list_launch(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                       hipStream_t s) {
+                       hipStream_t s, const long *range = nullptr) {
   if (n <= 0) return hipSuccess;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
   const long blocks_needed = (n + 255) / 256;
   const long cap = (long)device_cu_count() * 32;
   const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
-  pte_list_kernel<METHOD><<<g, 256, lds, s>>>(a, P, cells, n);
+  pte_list_kernel<METHOD><<<g, 256, lds, s>>>(a, P, cells, n, range);
   return hipGetLastError();
 }
 } // namespace kern

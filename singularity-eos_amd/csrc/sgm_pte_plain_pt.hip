@@ -8,4 +8,8 @@ hipError_t launch_pte_list_pt(const PteArgs &a, const MixParamsDev &P, const int
                              hipStream_t s) {
   return kern::list_launch<1>(a, P, cells, n, s);
 }
+hipError_t launch_pte_list_pt_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
+                                     const long *range, long nmax, hipStream_t s) {
+  return kern::list_launch<1>(a, P, cells, nmax, s, range);
+}
 } // namespace sgm

@@ -8,4 +8,8 @@ hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const i
                              hipStream_t s) {
   return kern::list_launch<0>(a, P, cells, n, s);
 }
+hipError_t launch_pte_list_rhot_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
+                                     const long *range, long nmax, hipStream_t s) {
+  return kern::list_launch<0>(a, P, cells, nmax, s, range);
+}
 } // namespace sgm

@@ -5,7 +5,7 @@ sgm_workload_*), so the same seed gives the same inputs everywhere.
 
   pte2     65,536 two-material cells: IdealGas(0.4, 7.176e6) + StiffGas(3.4, 3.6e7, 6e9, 0)
   table    256 x 512 rho-T table sampled from Gruneisen(rho0=2.7, C0=5.35e5, s1=1.34,
-           G0=2.0, b=0.48, T0=298, P0=0, Cv=9e6) on NQT-log10 grids
+           G0=2.0, Cv=9e6; unlisted s2=s3=b=P0=0, T0=298) on NQT-log10 grids
            rho in [1e-3, 1e2], T in [1e2, 1e6]; 2^24 log-uniform queries
   uneven   2^20 cells with 2..6 materials from a pool of six
 """
@@ -21,8 +21,15 @@ IDEAL_A = (0.4, 7.176e6)
 STIFF_B = (3.4, 3.6e7, 6e9, 0.0)
 IDEAL_M1 = (2.0 / 3.0, 3.1e7)
 # Gruneisen used for the table and for the pool's condensed slot:
-# (C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv)
-GRUN_AL = (5.35e5, 1.34, 0.0, 0.0, 2.0, 0.48, 2.7, 298.0, 0.0, 9.0e6)
+# (C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv). Parameters the workload does not
+# list are zero (T0 = 298 K). b = 0 matters: with the Al test's b = 0.48
+# (test_eos_gruneisen.cpp:282-291) the Hugoniot term P_H (1 - Gamma rho eta /
+# (2 rho0)) turns negative above rho ~ 8.2 g/cc, inside the table's range, the
+# table's density of minimum pressure becomes ~10.9 g/cc at every T, and
+# the closure bounds every table material above it: no table cell can reach
+# equilibrium.
+GRUN_C1 = (5.35e5, 1.34, 0.0, 0.0, 2.0, 0.0, 2.7, 298.0, 0.0, 9.0e6)
+GRUN_AL_TEST_B = 0.48
 # Vinet Cu-like: rho0, T0, B0, BP0, A0, Cv0, E0, S0
 VINET_CU = (8.93, 298.0, 1.37e12, 5.0, 5.19245e-5, 3.83e6, 0.0, 5.05e8)
 TABLE_NRHO, TABLE_NT = 256, 512
@@ -31,7 +38,7 @@ TABLE_NRHO, TABLE_NT = 256, 512
 class GruneisenTable:
     """Host arrays of the Gruneisen-sampled rho-T table plus its descriptor."""
 
-    def __init__(self, nrho=TABLE_NRHO, nt=TABLE_NT, grun=GRUN_AL, lrho=None, lt=None):
+    def __init__(self, nrho=TABLE_NRHO, nt=TABLE_NT, grun=GRUN_C1, lrho=None, lt=None):
         from .nqt import log10 as nlog10
         self.lRhoMin, self.lRhoMax = lrho if lrho else (nlog10(1e-3), nlog10(1e2))
         self.lTMin, self.lTMax = lt if lt else (nlog10(1e2), nlog10(1e6))
@@ -59,15 +66,15 @@ class GruneisenTable:
         return d
 
 
-def pool_context(table=None):
+def pool_context(table=None, host_only=False):
     """Context with the six pool materials in slots 0..5:
     0 IdealGas(0.4), 1 IdealGas(2/3), 2 StiffGas water-like, 3 Gruneisen Al-like,
     4 Vinet Cu-like, 5 the Gruneisen-sampled table."""
-    ctx = Context(6)
+    ctx = Context(6, host_only=host_only)
     ctx.set_ideal(0, *IDEAL_A)
     ctx.set_ideal(1, *IDEAL_M1)
     ctx.set_stiff(2, *STIFF_B)
-    ctx.set_gruneisen(3, *GRUN_AL)
+    ctx.set_gruneisen(3, *GRUN_C1)
     ctx.set_vinet(4, *VINET_CU)
     table = table or GruneisenTable()
     ctx.set_table(5, table.desc())

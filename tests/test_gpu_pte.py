@@ -7,11 +7,8 @@ singularity-eos/example/pte_2mat.cpp:169-180. Every operation on the path is
 IEEE +,-,*,/,sqrt,floor or integer, compiled without FMA contraction on both
 sides, with exp/cbrt (Vinet) evaluated by the same routine on host and device.
 
-Cells holding the Gruneisen-sampled table rarely converge, on the GPU and in
-the oracle alike: the table spans densities above the Gruneisen model's
-rho_max (9.37 g/cc), where P(rho) turns over, so the table's rho(P_min) is
-about 11 g/cc and the solver cannot place that material (the reference's
-SetVfracFromT / ScaleDx limits). Convergence-rate checks exclude those cells.
+Convergence-rate checks count every cell; the few that stop at the iteration
+cap or the small-step bailout do so identically in the oracle.
 """
 import numpy as np
 import pytest
@@ -89,10 +86,10 @@ def pool(sgm, workloads, oracle, torch_gpu):
     wl = workloads
     ctx, table = wl.pool_context()
     t = table
-    otab, _, _ = oracle.gruneisen_table(wl.GRUN_AL, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
+    otab, _, _ = oracle.gruneisen_table(wl.GRUN_C1, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
                                         t.lTMax, t.nT)
     o = [oracle.ideal(*wl.IDEAL_A), oracle.ideal(*wl.IDEAL_M1), oracle.stiff(*wl.STIFF_B),
-         oracle.gruneisen(*wl.GRUN_AL), oracle.vinet(*wl.VINET_CU), otab.eos()]
+         oracle.gruneisen(*wl.GRUN_C1), oracle.vinet(*wl.VINET_CU), otab.eos()]
     return ctx, o, otab
 
 
@@ -145,14 +142,6 @@ def compare_uneven(g, r, a):
     assert np.array_equal(g["err"], r["err"])
 
 
-def table_free(a):
-    off = a["offsets"]
-    has = np.zeros(a["ncell"], dtype=bool)
-    cell = np.repeat(np.arange(a["ncell"]), np.diff(off))
-    np.logical_or.at(has, cell, a["mat_ids"] == 5)
-    return ~has
-
-
 @pytest.mark.parametrize("method", [0, 1])
 def test_c2_plain_and_batched_vs_oracle(sgm, workloads, oracle, pool, torch_gpu, method):
     ctx, o, _ = pool
@@ -174,7 +163,7 @@ def test_c2_default_tolerance_vs_oracle(sgm, workloads, oracle, pool, torch_gpu)
     g = run_plain(torch_gpu, ctx, a, 0, params)
     r = run_oracle(oracle, o, a, 0, params.as_oracle_array())
     compare_uneven(g, r, a)
-    assert g["conv"][table_free(a)].mean() > 0.95
+    assert g["conv"].mean() > 0.95
 
 
 def test_batched_graph_replay_and_autotune_invariance(sgm, workloads, pool, torch_gpu):
@@ -256,7 +245,7 @@ def test_c2_full_size_invariants(sgm, workloads, pool, torch_gpu):
     assert_identical(b, g, "full size batched vs plain")
     off = a["offsets"]
     conv = b["conv"].astype(bool)
-    assert conv[table_free(a)].mean() > 0.95
+    assert conv.mean() > 0.95
     cell = np.repeat(np.arange(a["ncell"]), np.diff(off))
     vsum = np.bincount(cell, weights=b["vfrac"])
     assert np.max(np.abs(vsum[conv] - 1.0)) < 1e-12

@@ -32,10 +32,10 @@ def setup(sgm, workloads, oracle, torch_gpu):
     wl = workloads
     ctx, table = wl.pool_context()
     t = table
-    otab, _, _ = oracle.gruneisen_table(wl.GRUN_AL, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
+    otab, _, _ = oracle.gruneisen_table(wl.GRUN_C1, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
                                         t.lTMax, t.nT)
     o = [oracle.ideal(*wl.IDEAL_A), oracle.ideal(*wl.IDEAL_M1), oracle.stiff(*wl.STIFF_B),
-         oracle.gruneisen(*wl.GRUN_AL), oracle.vinet(*wl.VINET_CU), otab.eos()]
+         oracle.gruneisen(*wl.GRUN_C1), oracle.vinet(*wl.VINET_CU), otab.eos()]
     return ctx, o, otab, table
 
 

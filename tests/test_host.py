@@ -106,7 +106,7 @@ def test_gruneisen_table_builder_matches_oracle(workloads, oracle):
     """The library's host-side table sampler and the oracle's agree bit for bit
     on every node and on the derived cold curves."""
     t = workloads.GruneisenTable(nrho=64, nt=96)
-    _, f, c = oracle.gruneisen_table(workloads.GRUN_AL, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
+    _, f, c = oracle.gruneisen_table(workloads.GRUN_C1, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
                                      t.lTMax, t.nT)
     for k in f:
         np.testing.assert_array_equal(t.f[k], f[k], err_msg=k)
