@@ -105,20 +105,20 @@ typedef struct sgm_table_desc {
 } sgm_table_desc;
 
 /* ---- context and materials --------------------------------------------- */
-/* replaces init_sg_eos (singularity-eos/eos/singularity_eos.hpp:27) */
+/* replaces init_sg_eos (singularity-eos/eos/singularity_eos.hpp:28) */
 int sgm_init(int nmat, sgm_ctx **ctx);
 /* A context with no device (no counterpart in the reference): material setters,
  * table queries and the seeded workload builders work on the host; every call
  * that would launch returns SGM_E_NOGPU. For input generation and host tests. */
 int sgm_init_host(int nmat, sgm_ctx **ctx);
-/* replaces finalize_sg_eos (singularity_eos.hpp:131) */
+/* replaces finalize_sg_eos (singularity_eos.hpp:134) */
 int sgm_finalize(sgm_ctx *ctx);
-/* replaces init_sg_IdealGas (singularity_eos.hpp:29) */
+/* replaces init_sg_IdealGas (singularity_eos.hpp:30) */
 int sgm_set_IdealGas(sgm_ctx *ctx, int matindex, double gm1, double Cv);
-/* replaces init_sg_StiffGas (singularity_eos.hpp:57) */
+/* replaces init_sg_StiffGas (singularity_eos.hpp:60) */
 int sgm_set_StiffGas(sgm_ctx *ctx, int matindex, double gm1, double Cv, double Pinf,
                      double qq);
-/* replaces init_sg_Gruneisen (singularity_eos.hpp:36); rho_max <= 0 selects
+/* replaces init_sg_Gruneisen (singularity_eos.hpp:36-39); rho_max <= 0 selects
  * the automatic 0.99*ComputeRhoMax of the reference constructor */
 int sgm_set_Gruneisen(sgm_ctx *ctx, int matindex, double C0, double s1, double s2, double s3,
                       double G0, double b, double rho0, double T0, double P0, double Cv,
@@ -127,7 +127,7 @@ int sgm_set_Gruneisen(sgm_ctx *ctx, int matindex, double C0, double s1, double s
  * d2to40: HOST array of 39 coefficients */
 int sgm_set_Vinet(sgm_ctx *ctx, int matindex, double rho0, double T0, double B0, double BP0,
                   double A0, double Cv0, double E0, double S0, const double *d2to40);
-/* replaces init_sg_SpinerDependsRhoT (singularity_eos.hpp:76) */
+/* replaces init_sg_SpinerDependsRhoT (singularity_eos.hpp:80) */
 int sgm_set_TableRhoT(sgm_ctx *ctx, int matindex, const sgm_table_desc *desc);
 /* host-side query: kind of a material, or SGM_E_MAT */
 int sgm_get_kind(const sgm_ctx *ctx, int matindex);
@@ -137,7 +137,7 @@ int sgm_get_kind(const sgm_ctx *ctx, int matindex);
 long sgm_table_query(const sgm_ctx *ctx, int matindex, int which, double *out);
 
 /* ---- vector EOS calls --------------------------------------------------- */
-/* replaces get_sg_PressureFromDensityInternalEnergy (singularity_eos.hpp:103) and
+/* replaces get_sg_PressureFromDensityInternalEnergy (singularity_eos.hpp:101-104) and
  * Variant::PressureFromDensityInternalEnergy(rhos, sies, pressures, num, lambdas)
  * (eos_variant.hpp:543-545). lambda: NULL or [n*lambda_stride] with
  * lambda[i*stride+0] = log10 rho, [+1] = log10 T (table EOS cache). */

@@ -1,6 +1,7 @@
-"""Writes workload_pte2.json: SHA-256 of every array of the 65,536-cell
-two-material workload (mt19937_64, seed 0x5EED0000) and the values of its
-first 16 cells. Regenerate only when the generator's specification changes."""
+"""Writes workload_pte2.json and workload_uneven.json: SHA-256 of every array
+of the 65,536-cell two-material workload (mt19937_64, seed 0x5EED0000) and of
+the 2^20-cell uneven workload (seed 0x5EED0002), plus the values of their
+first cells. Regenerate only when a generator's specification changes."""
 import hashlib
 import importlib
 import json
@@ -23,3 +24,18 @@ out = {
     "head": {k: [float(x) for x in a[k][: 32 if k != "sie_tot" else 16]] for k in keys},
 }
 json.dump(out, open(os.path.join(HERE, "workload_pte2.json"), "w"), indent=1)
+
+ctx, _ = wl.pool_context(host_only=True)
+n = 1 << 20
+a = wl.uneven(ctx, n)
+ctx.close()
+keys = ("offsets", "mat_ids", "rho", "vfrac", "sie", "temp", "press", "sie_tot")
+out = {
+    "ncell": n,
+    "seed": "0x5EED0002",
+    "nnz": int(a["nnz"]),
+    "sha256": {k: hashlib.sha256(np.ascontiguousarray(a[k]).tobytes()).hexdigest() for k in keys},
+    "head": {k: [float(x) if a[k].dtype.kind == "f" else int(x) for x in a[k][:24]]
+             for k in keys},
+}
+json.dump(out, open(os.path.join(HERE, "workload_uneven.json"), "w"), indent=1)

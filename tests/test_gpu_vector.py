@@ -215,3 +215,38 @@ def test_table_full_size_queries(setup, workloads, torch_gpu):
     # classified off-table and answered by extrapolation: exclude them
     inside = (T > 100.0 * 1.0001) & (T < 1e6 * 0.9999)  # table spans T in [1e2, 1e6]
     assert float(rel[inside].max()) < 1e-10
+
+
+def test_cpp_adapter_matches_python_path(sgm, workloads, torch_gpu, tmp_path):
+    """The C++ adapter program (Variant-style vector calls and PTESolver over
+    the two-material workload) produces the same bits as the same calls made
+    through the Python binding."""
+    import os
+    import subprocess
+    torch = torch_gpu
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "singularity-eos_amd", "sgm_adapter_demo")
+    out_file = str(tmp_path / "adapter.bin")
+    r = subprocess.run([exe, out_file], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out_file, dtype=np.float64).reshape(7, -1)
+    n = 65536
+    a = workloads.pte2(n)
+    ctx = sgm.Context(2)
+    ctx.set_ideal(0, *workloads.IDEAL_A)
+    ctx.set_stiff(1, *workloads.STIFF_B)
+    d = {k: torch.tensor(a[k], device="cuda") for k in ("rho", "vfrac", "sie", "temp", "press",
+                                                        "vfrac_tot", "sie_tot")}
+    P = torch.empty_like(d["rho"])
+    T = torch.empty_like(d["rho"])
+    assert ctx.vec_eval(1, sgm.F_P_RHO_SIE, d["rho"], d["sie"], P) == 0
+    assert ctx.vec_eval(1, sgm.F_T_RHO_SIE, d["rho"], d["sie"], T) == 0
+    off = torch.tensor(a["offsets"], device="cuda", dtype=torch.int32)
+    mat = torch.tensor(a["mat_ids"], device="cuda")
+    assert ctx.pte(0, n, 2 * n, off, mat, d["rho"], d["vfrac"], d["sie"], d["temp"],
+                   d["press"], d["vfrac_tot"], d["sie_tot"], params=sgm.mix_params(tight=True)) == 0
+    torch.cuda.synchronize()
+    ref = [P, T, d["rho"], d["vfrac"], d["sie"], d["temp"], d["press"]]
+    for i, x in enumerate(ref):
+        assert np.array_equal(got[i], x.cpu().numpy()), i
+    ctx.close()

@@ -112,3 +112,14 @@ def test_gruneisen_table_builder_matches_oracle(workloads, oracle):
         np.testing.assert_array_equal(t.f[k], f[k], err_msg=k)
     for k in c:
         np.testing.assert_array_equal(t.c[k], c[k], err_msg=k)
+
+
+def test_cpp_adapter_on_host_only_registry():
+    """include/sgm.hpp compiles into a host program (built by the package
+    Makefile) that sets materials and gets SGM_E_NOGPU from a launch on a
+    registry without a device."""
+    import subprocess
+    exe = os.path.join(ROOT, "singularity-eos_amd", "sgm_adapter_demo")
+    out = subprocess.run([exe, "--host"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "adapter host OK" in out.stdout

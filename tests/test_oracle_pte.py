@@ -1,0 +1,189 @@
+"""Pins the oracle's PTE closure (singularity-eos/closure/mixed_cell_models.hpp
+restated in oracle/sgo_pte.hpp) against what the mathematics fixes, on the
+CPU: closed-form equilibria (N ideal gases; IdealGas + StiffGas), the
+conservation and equilibrium invariants of test_pte.cpp:204-261 on the
+uneven 2..6-material workload, PTESolverRhoT vs PTESolverPT agreement, and a
+planted-mistake check showing the invariants catch a wrong answer."""
+import numpy as np
+import pytest
+
+FIELDS = ("rho", "vfrac", "sie", "temp", "press")
+
+
+def solve(oracle, method, eos, offsets, mat, a, params):
+    r = {k: np.array(a[k], dtype=np.float64) for k in FIELDS}
+    nf, conv, nit, res, err = oracle.pte_solve(method, eos, offsets, mat, r["rho"], r["vfrac"],
+                                               r["sie"], r["temp"], r["press"], a["vfrac_tot"],
+                                               a["sie_tot"], params)
+    r.update(conv=conv, niter=nit, err=err)
+    return r
+
+
+def ideal_gas_cells(n, gams, Cvs, seed):
+    """n cells of len(gams) ideal gases, out of equilibrium."""
+    rng = np.random.default_rng(seed)
+    k = len(gams)
+    al = rng.exponential(size=(n, k)) + 0.05
+    al /= al.sum(1, keepdims=True)
+    rho = np.exp(rng.normal(np.log(1e-2), 1.0, (n, k)))
+    T = rng.uniform(300, 3000, (n, k))
+    e = np.array(Cvs)[None, :] * T
+    P = (np.array(gams)[None, :] - 0.0) * rho * e
+    rb = rho * al
+    a = dict(rho=rho.ravel(), vfrac=al.ravel(), sie=e.ravel(), temp=T.ravel(), press=P.ravel(),
+             vfrac_tot=np.ones(n), sie_tot=(rb * e).sum(1) / rb.sum(1))
+    return a, rb
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_ideal_gas_mixture_closed_form(oracle, method):
+    """N ideal gases: T* = u / sum(rb Cv), P* = T* sum(rb (g-1) Cv),
+    alpha_m = rb_m (g_m - 1) Cv_m T* / P* (exact; test_pte_ideal.cpp's setting)."""
+    gm1 = [0.4, 2.0 / 3.0, 0.3, 1.1, 0.55]
+    Cv = [7.176e6, 3.1e7, 1.2e7, 5.0e6, 2.2e7]
+    n = 400
+    a, rb = ideal_gas_cells(n, gm1, Cv, 5)
+    eos = [oracle.ideal(g, c) for g, c in zip(gm1, Cv)]
+    k = len(gm1)
+    off = np.arange(0, k * n + 1, k, dtype=np.int32)
+    mat = np.tile(np.arange(k, dtype=np.int32), n)
+    p = oracle.tight_params()
+    if method == 1:
+        # PTESolverPT's volume residual uses pte_rel_tolerance_v, which the
+        # 1e-10 set of pte_2mat.cpp leaves at 1e-6: tighten it too here
+        p[4] = p[11] = 1e-10
+    r = solve(oracle, method, eos, off, mat, a, p)
+    assert r["conv"].all()
+    u = a["sie_tot"] * rb.sum(1)
+    T = u / (rb * np.array(Cv)).sum(1)
+    P = T * (rb * np.array(gm1) * np.array(Cv)).sum(1)
+    al = rb * np.array(gm1) * np.array(Cv) * T[:, None] / P[:, None]
+    assert np.max(np.abs(r["temp"].reshape(n, k) - T[:, None]) / T[:, None]) < 1e-9
+    assert np.max(np.abs(r["press"].reshape(n, k) - P[:, None]) / P[:, None]) < 1e-9
+    assert np.max(np.abs(r["vfrac"].reshape(n, k) - al) / al) < 1e-9
+
+
+def test_ideal_stiff_quadratic_closed_form(oracle, workloads):
+    """IdealGas(0.4) + StiffGas(q=0): a quadratic in alpha_B (the 65,536-cell
+    two-material workload's closure), at the 1e-10 tolerances."""
+    n = 4096
+    a = workloads.pte2(n)
+    eos = [oracle.ideal(*workloads.IDEAL_A), oracle.stiff(*workloads.STIFF_B)]
+    off = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    mat = np.tile(np.array([0, 1], dtype=np.int32), n)
+    r = solve(oracle, 0, eos, off, mat, a, oracle.tight_params())
+    assert r["conv"].all()
+    L = np.longdouble
+    rb = (a["rho"] * a["vfrac"]).reshape(n, 2).astype(L)
+    u = a["sie_tot"].astype(L) * rb.sum(1)
+    gA, CA, gB, CB, Pi = L(0.4), L(7.176e6), L(3.4), L(3.6e7), L(6e9)
+    aa, bb, K = gA * rb[:, 0] * CA, gB * rb[:, 1] * CB, rb[:, 0] * CA + rb[:, 1] * CB
+    qa, qb, qc = Pi * (aa + bb + K), -((aa + bb) * u + (bb + K) * Pi), bb * u
+    disc = np.sqrt(qb * qb - 4 * qa * qc)
+    x1, x2 = (-qb - disc) / (2 * qa), (-qb + disc) / (2 * qa)
+    x = np.where((x1 > 0) & (x1 < 1) & ((u - Pi * x1) > 0), x1, x2)
+    T = (u - Pi * x) / K
+    P = aa * T / (1 - x)
+    assert float(np.max(np.abs(r["temp"].reshape(n, 2)[:, 0] - T) / T)) < 1e-11
+    assert float(np.max(np.abs(r["vfrac"].reshape(n, 2)[:, 1] - x) / x)) < 1e-11
+    assert float(np.max(np.abs(r["press"].reshape(n, 2) - P[:, None]) / np.abs(P[:, None]))) < 1e-9
+
+
+@pytest.fixture(scope="module")
+def c2(oracle, workloads):
+    wl = workloads
+    ctx, t = wl.pool_context(host_only=True)
+    a = wl.uneven(ctx, 4096)
+    otab, _, _ = oracle.gruneisen_table(wl.GRUN_C1, t.lRhoMin, t.lRhoMax, t.nRho, t.lTMin,
+                                        t.lTMax, t.nT)
+    eos = [oracle.ideal(*wl.IDEAL_A), oracle.ideal(*wl.IDEAL_M1), oracle.stiff(*wl.STIFF_B),
+           oracle.gruneisen(*wl.GRUN_C1), oracle.vinet(*wl.VINET_CU), otab.eos()]
+    yield a, eos, otab
+    ctx.close()
+
+
+def invariant_errors(a, r):
+    off = a["offsets"]
+    n = a["ncell"]
+    cell = np.repeat(np.arange(n), np.diff(off))
+    conv = r["conv"].astype(bool)
+    rb_in = a["rho"] * a["vfrac"]
+    e = {}
+    e["vfrac_sum"] = np.abs(np.bincount(cell, weights=r["vfrac"], minlength=n) - 1.0)[conv]
+    e["mass"] = (np.abs(r["rho"] * r["vfrac"] - rb_in) / rb_in)[conv[cell]]
+    u_in = a["sie_tot"] * np.bincount(cell, weights=rb_in, minlength=n)
+    u_out = np.bincount(cell, weights=r["rho"] * r["vfrac"] * r["sie"], minlength=n)
+    e["energy"] = (np.abs(u_out - u_in) / np.abs(u_in))[conv]
+    Pm = np.bincount(cell, weights=r["press"] * r["vfrac"], minlength=n)
+    e["pressure"] = (np.abs(r["press"] - Pm[cell]) / np.abs(Pm[cell]))[conv[cell]]
+    Tm = np.bincount(cell, weights=r["temp"], minlength=n) / np.diff(off)
+    e["temperature"] = (np.abs(r["temp"] - Tm[cell]) / Tm[cell])[conv[cell]]
+    return {k: float(v.max()) for k, v in e.items()}
+
+
+def test_uneven_workload_invariants(oracle, sgm, c2):
+    """test_pte.cpp:204-261 on the 2..6-material workload: volume fractions sum
+    to vfrac_tot, partial densities and total energy are conserved, and P, T
+    are equal across materials, at the 1e-10 tolerances."""
+    a, eos, _ = c2
+    r = solve(oracle, 0, eos, a["offsets"], a["mat_ids"].astype(np.int32), a,
+              sgm.mix_params(tight=True).as_oracle_array())
+    assert r["conv"].mean() > 0.98
+    e = invariant_errors(a, r)
+    assert e["vfrac_sum"] < 1e-13 and e["mass"] < 1e-13
+    assert e["energy"] < 1e-9 and e["pressure"] < 1e-9 and e["temperature"] < 1e-15
+
+
+def test_invariants_catch_a_planted_mistake(oracle, sgm, c2):
+    """The invariant check fails when a result is perturbed the way a dropped
+    term or a wrong index would perturb it."""
+    a, eos, _ = c2
+    r = solve(oracle, 0, eos, a["offsets"], a["mat_ids"].astype(np.int32), a,
+              sgm.mix_params(tight=True).as_oracle_array())
+    bad = {k: v.copy() for k, v in r.items()}
+    c = int(np.flatnonzero(r["conv"])[0])
+    k0 = int(a["offsets"][c])
+    nm = int(a["offsets"][c + 1]) - k0
+    mm = k0 + int(np.argmax((r["rho"] * r["vfrac"] * np.abs(r["sie"]))[k0:k0 + nm]))
+    bad["sie"][mm] *= 1 + 1e-6           # energy of the dominant material off
+    bad["press"][k0 + 1] *= 1 + 1e-6     # pressure of another material off
+    e = invariant_errors(a, bad)
+    assert e["energy"] > 1e-9 and e["pressure"] > 1e-9
+
+
+def test_rhoT_and_PT_agree(oracle, sgm, c2):
+    """The two formulations reach the same equilibrium where both converge."""
+    a, eos, _ = c2
+    p = sgm.mix_params(tight=True).as_oracle_array()
+    mat = a["mat_ids"].astype(np.int32)
+    r0 = solve(oracle, 0, eos, a["offsets"], mat, a, p)
+    p1 = list(p)
+    p1[4] = p1[11] = 1e-10  # PTESolverPT's volume tolerance, tightened like the others
+    r1 = solve(oracle, 1, eos, a["offsets"], mat, a, p1)
+    both = (r0["conv"] & r1["conv"]).astype(bool)
+    assert both.mean() > 0.9
+    cell = np.repeat(np.arange(a["ncell"]), np.diff(a["offsets"]))
+    sel = both[cell]
+    for k in ("temp", "press", "vfrac"):
+        rel = np.abs(r0[k][sel] - r1[k][sel]) / np.abs(r0[k][sel])
+        assert np.percentile(rel, 99) < 1e-8, k
+
+
+def test_uneven_workload_fixture(workloads):
+    """The 2^20-cell uneven workload (mt19937_64, seed 0x5EED0002): SHA-256 of
+    every array, committed, and its first cells."""
+    import hashlib
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "workload_uneven.json")))
+    ctx, _ = workloads.pool_context(host_only=True)
+    a = workloads.uneven(ctx, fx["ncell"])
+    ctx.close()
+    for k, h in fx["sha256"].items():
+        assert hashlib.sha256(np.ascontiguousarray(a[k]).tobytes()).hexdigest() == h, k
+    for k, v in fx["head"].items():
+        np.testing.assert_array_equal(a[k][: len(v)], np.array(v))
+    nm = np.diff(a["offsets"])
+    frac = np.bincount(nm, minlength=7)[2:] / len(nm)
+    assert np.max(np.abs(frac - [0.40, 0.25, 0.15, 0.12, 0.08])) < 0.003
