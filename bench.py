@@ -43,6 +43,44 @@ def bytes_per_cell(nm):
     return 8 * 4 * nm + 16 + 4 + nm + 8 * 5 * nm + 4
 
 
+def serve_bench(torch, ctx, a, params, nreq):
+    """Serving loop on the same pool and closure workload: 64 clients, request
+    sizes log-uniform in [1e3, 1e6], 70% vector {P, T}(rho, e) / 30% closure.
+    Capacity = all requests at t = 0; then Poisson arrivals at 50/80/95% of the
+    served capacity. Served = continuous batching over 4 lanes; plain = one
+    request at a time through the plain per-call kernels. Host arrays in, host
+    arrays out: cells/s here include the PCIe copies."""
+    serving = importlib.import_module("singularity-eos_amd.serving")
+    pools = serving.Pools(torch, a)
+    n = a["ncell"]
+    out = {}
+    sat = serving.make_trace(nreq, np.inf, 0x5EED0003, pool_cells=n)
+    cap = {}
+    for mode, plain in (("served", False), ("plain", True)):
+        r = serving.run(torch, ctx, pools, sat, plain=plain, params=params, ring_bytes=1 << 30)
+        s = serving.summarize(r)
+        cap[mode] = nreq / r["makespan"]
+        out[f"{mode}_capacity"] = dict(s, requests_per_s=cap[mode])
+        log(f"[bench] serving {mode} capacity: {s['cells_per_s']:.4g} cells/s, "
+            f"{cap[mode]:.3g} req/s, p99 {s['p99_ms']:.1f} ms")
+    for load in (0.5, 0.8, 0.95):
+        tr = serving.make_trace(nreq, load * cap["served"], 0x5EED0004, pool_cells=n)
+        r = serving.run(torch, ctx, pools, tr, plain=False, params=params, ring_bytes=1 << 30)
+        s = serving.summarize(r)
+        out[f"served_at_{int(load * 100)}pct"] = dict(s, offered_req_per_s=load * cap["served"])
+        log(f"[bench] served at {load:.0%}: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms")
+    # plain at its own 50% load, and at the served loop's 50% load
+    for tag, rate in (("plain_at_50pct", 0.5 * cap["plain"]),
+                      ("plain_at_served_50pct", 0.5 * cap["served"])):
+        tr = serving.make_trace(nreq, rate, 0x5EED0004, pool_cells=n)
+        r = serving.run(torch, ctx, pools, tr, plain=True, params=params, ring_bytes=1 << 30)
+        s = serving.summarize(r)
+        out[tag] = dict(s, offered_req_per_s=rate)
+        log(f"[bench] {tag}: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms")
+    out["served_vs_plain_capacity"] = cap["served"] / cap["plain"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +90,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--copy-cal", action="store_true",
                     help="end with one 1 GiB device copy (scale for FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -203,6 +243,9 @@ def main():
         vms = e0.elapsed_time(e1)
         extras["vector_P_rho_sie_GBs"] = nq * 24 / (vms * 1e-3) / 1e9
 
+    if not args.no_serve:
+        extras["serving"] = serve_bench(torch, ctx, a, params, args.serve_requests)
+
     cpu = None
     flops_per_cell = None
     if not args.no_cpu:
@@ -232,14 +275,22 @@ def main():
                         params.as_oracle_array(), nthreads=nthreads)
             return time.perf_counter() - t0
 
-        cpu_run(min(4096, ns), threads)  # warm-up
-        cdt = cpu_run(ns, threads)
+        cpu_run(ns, threads)  # warm-up
+        cdt = float(np.median([cpu_run(ns, threads) for _ in range(10)]))
         n1 = min(8192, ns)
-        c1dt = cpu_run(n1, 1)
+        c1dt = float(np.median([cpu_run(n1, 1) for _ in range(3)]))
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": ns / cdt, "unit": "cells/s", "cores": threads, "kind": "port",
                "sample": f"first {ns} cells of the same 2^20-cell uneven workload, "
-                         f"PTESolverRhoT at 1e-10 tol, OpenMP over cells ({cdt:.2f} s); "
-                         f"1 thread on the first {n1} cells: {n1 / c1dt:.4g} cells/s",
+                         f"PTESolverRhoT at 1e-10 tol, OpenMP over cells on {threads} of "
+                         f"{os.cpu_count()} host threads ({model}), median of 10 runs "
+                         f"({cdt:.3f} s each) after a warm-up; 1 thread on the first {n1} "
+                         f"cells: {n1 / c1dt:.4g} cells/s",
                "single_core_value": n1 / c1dt}
         log(f"[bench] cpu baseline: {cpu['value']:.4g} cells/s on {threads} threads")
         fl = O.count_pte_flops(0, specs, oo, me, a["rho"][:k], a["vfrac"][:k], a["temp"][:k],

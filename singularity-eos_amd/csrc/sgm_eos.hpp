@@ -506,7 +506,7 @@ struct EP {
   Real e, P, cs2;
 };
 template <bool WITH_CS>
-SGM_NOINL EP F_DT(const Real *p, const Real *coef, int nskip, Real rho, Real T) {
+SGM_HD EP F_DT(const Real *p, const Real *coef, int nskip, Real rho, Real T) {
   const Real rho0 = p[0], T0 = p[1], B0 = p[2], A0 = p[4], Cv0 = p[5], E0 = p[6];
   const Real *d2tod40 = coef;
   const Real *VIP = coef + 39;
@@ -586,6 +586,29 @@ SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   case KIND_GRUN: return grun::e_rT(e.p, rho, T);
   case KIND_VINET: return vinet::e_rT(e.p, e.coef, e.nvinet_skip, rho, T);
   default: return table::e_rT(*e.tab, rho, T, lam);
+  }
+}
+// e(rho, T) and, for models whose preferred input is (rho, T) (Vinet, table),
+// P(rho, T) at the same point from one evaluation: one Vinet F_DT, one table
+// index and weight computation. The values equal separate eos_e_rT and
+// eos_P_rT calls bit for bit. For (rho, e)-preferred models P_T is 0 and the
+// caller evaluates P(rho, e).
+SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
+  switch (e.kind) {
+  case KIND_VINET: {
+    const vinet::EP o = vinet::F_DT<false>(e.p, e.coef, e.nvinet_skip, rho, T);
+    P_T = o.P;
+    return o.e;
+  }
+  case KIND_TABLE: {
+    const TableDev &t = *e.tab;
+    const Real lRho = table::lRho_(t, rho), lT = table::lT_(t, T);
+    lam.set(lRho, lT);
+    const int w = table::loc_rho_T(t, lRho, lT);
+    P_T = table::P_from(t, rho, lRho, T, lT, w);
+    return table::sie_from(t, lRho, T, lT, w);
+  }
+  default: P_T = 0.0; return eos_e_rT(e, rho, T, lam);
   }
 }
 SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {

@@ -294,6 +294,12 @@ struct PteBase {
     if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
     return eos_P_rT(e, r, T, lam[m]);
   }
+  // the same, with P(rho, T) already evaluated beside e (eos_e_rT_with_P)
+  SGM_HD Real PressureFromPreferredP(int m, Real r, Real s, Real P_T) {
+    const EosDev &e = *eos[m];
+    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
+    return P_T;
+  }
   // :599-620
   SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
     const EosDev &e = *eos[m];
@@ -366,11 +372,12 @@ struct PteBase {
     Tnorm = Tguess;
 #pragma unroll
     for (int m = 0; m < NM; m++) {
-      sie[m] = eos_e_rT(*eos[m], rho[m], Tguess, lam[m]);
+      Real P_T;
+      sie[m] = eos_e_rT_with_P(*eos[m], rho[m], Tguess, lam[m], P_T);
       if ((preferred_input(*eos[m]) & tq::pressure) && !bad_value(press[m])) {
         press[m] = robust::ratio(press[m], uscale);
       } else {
-        press[m] = robust::ratio(PressureFromPreferred(m, rho[m], Tguess, sie[m]), uscale);
+        press[m] = robust::ratio(PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
       }
     }
 #pragma unroll
@@ -495,12 +502,13 @@ struct PteRhoT : PteBase<NM, NM> {
       const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
       const Real vf_pert = vfrac[m] + dv;
       const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
-      Real e_pert = eos_e_rT(*eos[m], rho_pert, Tcur, lam[m]);
-      Real p_pert = robust::ratio(B::PressureFromPreferred(m, rho_pert, Tcur, e_pert), uscale);
+      Real P_T;
+      Real e_pert = eos_e_rT_with_P(*eos[m], rho_pert, Tcur, lam[m], P_T);
+      Real p_pert = robust::ratio(B::PressureFromPreferredP(m, rho_pert, e_pert, P_T), uscale);
       dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
       dedv[m] = robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dv);
-      e_pert = eos_e_rT(*eos[m], rho[m], Tpert, lam[m]);
-      p_pert = robust::ratio(B::PressureFromPreferred(m, rho[m], Tpert, e_pert), uscale);
+      e_pert = eos_e_rT_with_P(*eos[m], rho[m], Tpert, lam[m], P_T);
+      p_pert = robust::ratio(B::PressureFromPreferredP(m, rho[m], e_pert, P_T), uscale);
       dpdT[m] = robust::ratio((p_pert - press[m]), dT);
       dedT_sum += robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dT);
     }
@@ -581,10 +589,11 @@ struct PteRhoT : PteBase<NM, NM> {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
-      Real um = rhobar[m] * eos_e_rT(*eos[m], rho[m], Tc, lam[m]);
+      Real P_T;
+      Real um = rhobar[m] * eos_e_rT_with_P(*eos[m], rho[m], Tc, lam[m], P_T);
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
-      press[m] = robust::ratio(B::PressureFromPreferred(m, rho[m], Tc, sie[m]), uscale);
+      press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
     }
     Residual();
     return B::ResidualNorm();
