@@ -22,6 +22,7 @@ typedef struct ihipStream_t *sgm_stream_t; /* == hipStream_t */
 typedef struct sgm_ctx sgm_ctx;
 typedef struct sgm_plan sgm_plan;
 typedef struct sgm_server sgm_server;
+typedef struct sgm_mlp sgm_mlp;
 
 enum {
   SGM_OK = 0,
@@ -279,6 +280,58 @@ int sgm_server_stats_get(sgm_server *server, sgm_server_stats *out);
 /* drains the queue and stops the scheduler */
 int sgm_server_destroy(sgm_server *server);
 
+
+
+/* ---- learned initial guess ------------------------------------------------
+ * No reference counterpart. A 64 -> 128 -> 128 -> 128 -> 16 MLP (GELU; BF16
+ * weights, FP32 accumulation on the matrix cores) predicts a cell's
+ * equilibrium volume fractions (masked softmax over its materials) and ln T
+ * from its inputs. Features per material slot s < 6: one-hot(material id),
+ * alpha, ln rho, ln T; per cell: asinh(sie_tot / 1e10), ln(rho_total). Cells
+ * with other material counts (or ids >= 6) get no guess (guess_T = NaN).
+ * Training runs on the device (forward, loss, backward, Adam kernels). */
+int sgm_mlp_create(uint64_t seed, sgm_mlp **mlp);
+int sgm_mlp_destroy(sgm_mlp *mlp);
+int sgm_mlp_nparam(void);
+/* host copies of the FP32 master parameters: out and/or in may be NULL */
+int sgm_mlp_params(sgm_mlp *mlp, float *out, const float *in);
+/* feature standardization (host arrays [64]) and the ln T standardization */
+int sgm_mlp_set_norm(sgm_mlp *mlp, const float *mu, const float *sigma, float lnT_mu,
+                     float lnT_sigma);
+/* features of CSR cells: raw_out [ncell*64] FP32 (may be NULL), X_out
+ * [ncell*64] bf16 standardized (may be NULL), nmat_out [ncell] (0 = no guess) */
+int sgm_mlp_features(sgm_mlp *mlp, long ncell, long nnz, const int *offsets,
+                     const uint8_t *mat_ids, const double *rho, const double *vfrac,
+                     const double *temp, const double *sie_tot, float *raw_out, void *X_out,
+                     int *nmat_out, sgm_stream_t stream);
+/* training targets [ncell*8] from a solved closure (alphas, standardized ln T, nmat) */
+int sgm_mlp_targets(sgm_mlp *mlp, long ncell, long nnz, const int *offsets,
+                    const double *vfrac, const double *temp, const double *vfrac_tot,
+                    const int *nmat, const int *converged, float *targets, sgm_stream_t stream);
+/* loss gradient over rows idx[0..batch) of X/targets (no update); grad_out
+ * [nparam], loss_out [batch] (either may be NULL) */
+int sgm_mlp_gradients(sgm_mlp *mlp, const void *X, const float *targets, const int *idx,
+                      long batch, float wT, float *grad_out, float *loss_out,
+                      sgm_stream_t stream);
+/* one Adam step (lr) on the same loss */
+int sgm_mlp_train_step(sgm_mlp *mlp, const void *X, const float *targets, const int *idx,
+                       long batch, float wT, float lr, float *loss_out, sgm_stream_t stream);
+/* network outputs [batch*16] for rows idx of X */
+int sgm_mlp_forward(sgm_mlp *mlp, const void *X, const int *idx, long batch, float *Y_out,
+                    sgm_stream_t stream);
+/* guesses for a closure call: guess_vfrac [nnz], guess_T [ncell] */
+int sgm_mlp_guess(sgm_mlp *mlp, sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                  const uint8_t *mat_ids, const double *rho, const double *vfrac,
+                  const double *temp, const double *vfrac_tot, const double *sie_tot,
+                  double *guess_vfrac, double *guess_T, sgm_stream_t stream);
+/* sgm_plan_execute starting each cell from the guess where it lowers the
+ * initial residual of the closure, from the solver's own guess elsewhere
+ * (cells with guess_T NaN or <= 0 always use their own) */
+int sgm_plan_execute_guided(sgm_plan *plan, double *rho, double *vfrac, double *sie,
+                            double *temp, double *press, const double *vfrac_tot,
+                            const double *sie_tot, const double *guess_vfrac,
+                            const double *guess_T, const sgm_mix_params *params,
+                            sgm_pte_status status, int *used_guess, sgm_stream_t stream);
 
 /* ---- defaults and misc -------------------------------------------------- */
 void sgm_mix_params_default(sgm_mix_params *p);

@@ -124,6 +124,19 @@ _SIGS = {
     "sgm_version": (C.c_char_p, []),
     "sgm_build_gruneisen_table": (ci, [cd] * 10 + [cd, cd, ci, cd, cd, ci] + [vp] * 10),
     "sgm_workload_pte2": (ci, [C.c_uint64, cl, vp, vp, vp, vp, vp, vp]),
+    "sgm_mlp_create": (ci, [C.c_uint64, C.POINTER(vp)]),
+    "sgm_mlp_destroy": (ci, [vp]),
+    "sgm_mlp_nparam": (ci, []),
+    "sgm_mlp_params": (ci, [vp, vp, vp]),
+    "sgm_mlp_set_norm": (ci, [vp, vp, vp, C.c_float, C.c_float]),
+    "sgm_mlp_features": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "sgm_mlp_targets": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "sgm_mlp_gradients": (ci, [vp, vp, vp, vp, cl, C.c_float, vp, vp, vp]),
+    "sgm_mlp_train_step": (ci, [vp, vp, vp, vp, cl, C.c_float, C.c_float, vp, vp]),
+    "sgm_mlp_forward": (ci, [vp, vp, vp, cl, vp, vp]),
+    "sgm_mlp_guess": (ci, [vp, vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "sgm_plan_execute_guided": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                     C.POINTER(MixParams), PteStatus, vp, vp]),
     "sgm_server_create": (ci, [vp, C.POINTER(ServerConfig), C.POINTER(MixParams),
                                C.POINTER(vp)]),
     "sgm_server_submit": (ci, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
@@ -290,6 +303,16 @@ class Plan:
                                       ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm),
                                       ptr(lam), C.byref(params) if params is not None else None,
                                       st, stream)
+
+    def execute_guided(self, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, guess_vfrac,
+                       guess_T, params=None, conv=None, niter=None, resid=None, err=None,
+                       used=None, stream=0):
+        st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
+        return lib().sgm_plan_execute_guided(self.h, ptr(rho), ptr(vfrac), ptr(sie), ptr(temp),
+                                             ptr(press), ptr(vfrac_tot), ptr(sie_tot),
+                                             ptr(guess_vfrac), ptr(guess_T),
+                                             C.byref(params) if params is not None else None,
+                                             st, ptr(used), stream)
 
     def execute_bin(self, b, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm=None,
                     lam=None, params=None, conv=None, niter=None, resid=None, err=None,

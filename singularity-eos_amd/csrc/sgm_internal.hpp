@@ -62,16 +62,19 @@ hipError_t launch_rhoe_PT(const EosDev &e_host, const EosDev *d_eos, int mat, co
                           double *lambda, int *errs, hipStream_t s);
 
 struct PteArgs {
-  const EosDev *d_eos;
-  int nreg;
-  long ncell, nnz;
-  const int *offsets;
-  const uint8_t *mat_ids;
-  double *rho, *vfrac, *sie, *temp, *press;
-  const double *vfrac_tot, *sie_tot, *Tnorm;
-  double *lambda;
-  int *conv, *niter, *err;
-  double *resid;
+  const EosDev *d_eos = nullptr;
+  int nreg = 0;
+  long ncell = 0, nnz = 0;
+  const int *offsets = nullptr;
+  const uint8_t *mat_ids = nullptr;
+  double *rho = nullptr, *vfrac = nullptr, *sie = nullptr, *temp = nullptr, *press = nullptr;
+  const double *vfrac_tot = nullptr, *sie_tot = nullptr, *Tnorm = nullptr;
+  double *lambda = nullptr;
+  int *conv = nullptr, *niter = nullptr, *err = nullptr;
+  double *resid = nullptr;
+  // learned initial guess (batched kernels only): per entry vfrac, per cell T
+  const double *guess_vfrac = nullptr, *guess_T = nullptr;
+  int *used_guess = nullptr;
 };
 hipError_t launch_pte_plain_rhot(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
 hipError_t launch_pte_plain_pt(const PteArgs &a, const MixParamsDev &P, hipStream_t s);

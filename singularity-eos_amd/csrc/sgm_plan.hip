@@ -23,7 +23,7 @@ struct Bin {
 };
 
 struct GraphKey {
-  const void *ptrs[13];
+  const void *ptrs[16];
   MixParamsDev P;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -338,21 +338,23 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
   return rc;
 }
 
-int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
-                     double *press, const double *vfrac_tot, const double *sie_tot,
-                     const double *Tnorm, double *lambda, const sgm_mix_params *params,
-                     sgm_pte_status status, sgm_stream_t stream) {
-  if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot) return SGM_E_ARG;
-  hipStream_t s = (hipStream_t)stream;
+} // extern "C"
+
+namespace {
+int execute_common(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                   double *press, const double *vfrac_tot, const double *sie_tot,
+                   const double *Tnorm, double *lambda, const sgm_mix_params *params,
+                   sgm_pte_status status, const double *guess_vfrac, const double *guess_T,
+                   int *used_guess, hipStream_t s) {
   const MixParamsDev P = params_dev(params);
   GraphKey k;
   std::memset(&k, 0, sizeof(k));
-  const void *ptrs[13] = {rho,    vfrac,          sie,         temp, status.residual,
-                          press,  vfrac_tot,      sie_tot,     Tnorm,
-                          lambda, status.converged, status.niter, status.err};
-  for (int i = 0; i < 13; ++i) k.ptrs[i] = ptrs[i];
+  const void *ptrs[16] = {rho,         vfrac,        sie,          temp,
+                          status.residual, press,    vfrac_tot,    sie_tot,
+                          Tnorm,       lambda,       status.converged, status.niter,
+                          status.err,  guess_vfrac,  guess_T,      used_guess};
+  for (int i = 0; i < 16; ++i) k.ptrs[i] = ptrs[i];
   k.P = P;
-  // status.residual participates through the captured kernel arguments
   if (!p->have_key || !(p->key == k) || !p->exec) {
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
@@ -360,6 +362,9 @@ int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, doubl
     p->graph = nullptr;
     PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
                           status);
+    a.guess_vfrac = guess_vfrac;
+    a.guess_T = guess_T;
+    a.used_guess = used_guess;
     if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return SGM_E_HIP;
     const hipError_t le = launch_bins(p, a, P, p->cap_stream, true);
@@ -378,6 +383,29 @@ int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, doubl
     p->have_key = true;
   }
   return hipGraphLaunch(p->exec, s) == hipSuccess ? SGM_OK : SGM_E_HIP;
+}
+} // namespace
+
+extern "C" {
+int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                     double *press, const double *vfrac_tot, const double *sie_tot,
+                     const double *Tnorm, double *lambda, const sgm_mix_params *params,
+                     sgm_pte_status status, sgm_stream_t stream) {
+  if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot) return SGM_E_ARG;
+  return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
+                        params, status, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int sgm_plan_execute_guided(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                            double *press, const double *vfrac_tot, const double *sie_tot,
+                            const double *guess_vfrac, const double *guess_T,
+                            const sgm_mix_params *params, sgm_pte_status status,
+                            int *used_guess, sgm_stream_t stream) {
+  if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot ||
+      !guess_vfrac || !guess_T)
+    return SGM_E_ARG;
+  return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, nullptr, nullptr,
+                        params, status, guess_vfrac, guess_T, used_guess, (hipStream_t)stream);
 }
 
 // Launches one bin's kernel directly (no graph) — for per-kernel timing.

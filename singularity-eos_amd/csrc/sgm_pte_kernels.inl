@@ -142,6 +142,42 @@ __global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev 
   }
 }
 
+// Learned initial guess: restart the cell from the predicted volume
+// fractions (densities follow from the fixed partial densities) and
+// temperature (as Tnorm), and keep that start only if its initial residual is
+// below the solver's own start.
+template <int NM, class Sys>
+__device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], DriverState &d,
+                                                 CellStatus &st, const EosDev *lds,
+                                                 const PteArgs &a, const MixParamsDev &P, long c,
+                                                 int off) {
+  const Real gT = a.guess_T[c];
+  int used = 0;
+  if (gT > 0) {  // NaN: no guess for this cell
+    const Real err_plain = d.err;
+    load_cell<NM>(s, Tin, lds, a, P, c, off);
+    bool ok = true;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real g = a.guess_vfrac[off + m];
+      ok = ok && (g > 0);
+      const Real rb = s.rho[m] * s.vfrac[m];
+      s.vfrac[m] = g;
+      s.rho[m] = rb / g;
+    }
+    s.Tnorm = gT;
+    if (ok) {
+      pte_begin<NM>(s, Tin, d, st);
+      used = d.err < err_plain ? 1 : 0;
+    }
+    if (!used) {
+      load_cell<NM>(s, Tin, lds, a, P, c, off);
+      pte_begin<NM>(s, Tin, d, st);
+    }
+  }
+  if (a.used_guess) a.used_guess[c] = used;
+}
+
 // Batched bin kernel with lane refill: every lane owns one cell and runs one
 // Newton iteration of it per trip of the loop; a lane whose cell has finished
 // stores it and takes the next cell of the bin at once, so a wave never idles
@@ -201,6 +237,7 @@ __global__ void __launch_bounds__(256) pte_bin_kernel(PteArgs a, MixParamsDev P,
             Real Tin[NM];
             load_cell<NM>(s, Tin, lds, a, P, c, off);
             pte_begin<NM>(s, Tin, d, st);
+            if (a.guess_T) start_from_guess<NM>(s, Tin, d, st, lds, a, P, c, off);
             active = true;
           } else if (c >= 0 && c < a.ncell) {
             mark_bad(a, c);

@@ -1,0 +1,116 @@
+"""Learned initial guess: the device MLP kernels against the FP64 numpy
+restatement (oracle/mlp_ref.py) — forward outputs and loss gradients — and
+the trained guess in the closure: fewer Newton iterations on held-out cells
+at the same 1e-10 accuracy, falling back to the solver's own guess where the
+learned one does not lower the initial residual."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L(sgm, torch_gpu):
+    return importlib.import_module("singularity-eos_amd.learned")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return importlib.import_module("mlp_ref")
+
+
+def random_batch(torch, ref, B, seed):
+    rng = np.random.default_rng(seed)
+    Xf = rng.normal(0, 1, (B, 64)).astype(np.float32)
+    Xf[:, 56:] = 0
+    bits = (Xf.view(np.uint32) + 0x7FFF + ((Xf.view(np.uint32) >> 16) & 1)) >> 16
+    bits = bits.astype(np.uint16)
+    X = torch.tensor(bits.view(np.int16), device="cuda")
+    T = np.zeros((B, 8), dtype=np.float32)
+    nm = rng.integers(2, 7, B)
+    for r in range(B):
+        a = rng.exponential(size=nm[r])
+        T[r, :nm[r]] = a / a.sum()
+        T[r, 6] = rng.normal()
+        T[r, 7] = nm[r]
+    return X, ref.bf16_bits_to_f64(bits), T
+
+
+def test_forward_matches_fp64_reference(torch_gpu, L, ref):
+    torch = torch_gpu
+    mlp = L.MLP(seed=3)
+    B = 200  # a partial last tile of 64 rows
+    X, Xr, _ = random_batch(torch, ref, B, 1)
+    idx = torch.arange(B, dtype=torch.int32, device="cuda")
+    Y = mlp.forward(torch, X, idx).cpu().numpy()
+    Yr, _ = ref.forward(mlp.params(), Xr)
+    err = np.abs(Y[:, :7] - Yr[:, :7])
+    assert err.max() < 0.05 * max(1.0, np.abs(Yr[:, :7]).max()), err.max()
+    assert np.median(err) < 5e-3
+    mlp.close()
+
+
+def test_gradients_match_fp64_reference(torch_gpu, L, ref):
+    torch = torch_gpu
+    mlp = L.MLP(seed=4)
+    B = 256
+    X, Xr, T = random_batch(torch, ref, B, 2)
+    tgt = torch.tensor(T, device="cuda")
+    idx = torch.arange(B, dtype=torch.int32, device="cuda")
+    g, loss = mlp.gradients(torch, X, tgt, idx)
+    g = g.cpu().numpy().astype(np.float64)
+    gr, Lr = ref.gradients(mlp.params(), Xr, T)
+    np.testing.assert_allclose(loss.cpu().numpy(), Lr, rtol=0.05, atol=1e-3)
+    cos = g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr))
+    assert cos > 0.999, cos
+    assert abs(np.linalg.norm(g) / np.linalg.norm(gr) - 1) < 0.03
+    for a0, a1 in ((ref.OW1, ref.OB1), (ref.OW2, ref.OB2), (ref.OW3, ref.OB3),
+                   (ref.OW4, ref.OB4), (ref.OB4, ref.NPARAM)):
+        d = np.linalg.norm(g[a0:a1] - gr[a0:a1]) / np.linalg.norm(gr[a0:a1])
+        assert d < 0.06, (a0, d)
+    # a planted mistake in the reference (a dropped GELU' factor) is caught
+    gz = ref.gelu_d
+    ref.gelu_d = lambda z: np.ones_like(z)
+    try:
+        bad, _ = ref.gradients(mlp.params(), Xr, T)
+    finally:
+        ref.gelu_d = gz
+    cos_bad = g @ bad / (np.linalg.norm(g) * np.linalg.norm(bad))
+    assert cos_bad < 0.99
+    mlp.close()
+
+
+def test_training_reduces_newton_iterations(sgm, workloads, torch_gpu, L):
+    torch = torch_gpu
+    ctx, _ = workloads.pool_context()
+    params = sgm.mix_params(tight=True)
+    train_set = workloads.uneven(ctx, 1 << 17, seed=0x5EED0000 + 40)
+    held = workloads.uneven(ctx, 1 << 15, seed=0x5EED0000 + 41)
+    mlp = L.MLP(seed=5)
+    X, tgt, rows = L.build_training_set(torch, mlp, ctx, train_set, params)
+    hist = L.train(torch, mlp, X, tgt, rows, steps=800, batch=4096)
+    assert hist[-1][1] < 0.5 * hist[0][1]
+    d = L.to_device(torch, held)
+    n, nnz = held["ncell"], held["nnz"]
+    base, conv0, it0, _ = L.solve(torch, ctx, d, n, nnz, params)
+    gv, gT = mlp.guess(torch, ctx, d, n, nnz)
+    out, conv1, it1, used = L.solve(torch, ctx, d, n, nnz, params, guess=(gv, gT))
+    it0, it1 = it0.cpu().numpy(), it1.cpu().numpy()
+    c0, c1 = conv0.cpu().numpy().astype(bool), conv1.cpu().numpy().astype(bool)
+    used = used.cpu().numpy().astype(bool)
+    assert used.mean() > 0.3
+    assert c1.mean() >= c0.mean() - 0.002
+    both = c0 & c1
+    assert it1[both].mean() < it0[both].mean()
+    # equal accuracy: the two starts reach the same equilibrium
+    cell = np.repeat(np.arange(n), np.diff(held["offsets"]))
+    sel = both[cell]
+    for k in ("temp", "press", "vfrac"):
+        a0 = base[k].cpu().numpy()[sel]
+        a1 = out[k].cpu().numpy()[sel]
+        rel = np.abs(a1 - a0) / np.maximum(np.abs(a0), 1e-300)
+        assert np.percentile(rel, 99.9) < 1e-7, (k, np.percentile(rel, 99.9))
+    mlp.close()
+    ctx.close()
