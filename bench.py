@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-reorder", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--copy-cal", action="store_true",
                     help="end with one 1 GiB device copy (scale for FETCH/WRITE counters)")
@@ -140,14 +141,18 @@ def main():
                           st, params=params, conv=conv, niter=niter, stream=stream)
         assert rc == 0, rc
 
+    def timed(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    # the headline: cells in material-signature order
+    dt = timed(args.steps)
     cells_per_s = n / dt
     conv_frac = float(conv.float().mean().item())
     mean_iter = float(niter.float().mean().item())
@@ -179,7 +184,17 @@ def main():
         nm = b["nmat"]
         per_bin.append(dict(nmat=nm, cells=cells, ms=ms, block=b["block"],
                             cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
-    extras = {"bins": per_bin, "converged_frac": conv_frac, "mean_newton_iters": mean_iter}
+    extras = {"bins": per_bin, "converged_frac": conv_frac, "mean_newton_iters": mean_iter,
+              "max_newton_iters": int(niter.max().item())}
+    if not args.no_reorder:
+        # the plan reused step after step: each bin ordered by the previous
+        # step's iteration counts (the slowest cells spread one per wave)
+        plan.reorder(niter, stream=stream)
+        for _ in range(args.warmup):
+            step()
+        dt_r = timed(args.steps)
+        extras["batched_cost_ordered_cells_per_s"] = n / dt_r
+        log(f"[bench] batched, cost-ordered bins: {dt_r * 1e3:.3f} ms/step")
     if not args.no_extras:
         # plain per-call path, same cells
         restore()

@@ -210,6 +210,11 @@ int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, do
                          const double *sie_tot, const double *Tnorm, double *lambda,
                          const sgm_mix_params *params, sgm_pte_status status,
                          sgm_stream_t stream);
+/* Reorders cells within each material count so that cells whose cost (e.g.
+ * the niter status of a previous execute) reaches 4x the mean come first:
+ * the slowest cells start with the bulk instead of trailing it. Results do
+ * not depend on the order. Synchronizes the stream. */
+int sgm_plan_reorder(sgm_plan *plan, const int *cost, sgm_stream_t stream);
 /* host: per-bin description, out[6*i..] = {nmat, ncells, block, cells_per_wave, grid, start} */
 int sgm_plan_describe(const sgm_plan *plan, long *out, int max_bins);
 int sgm_plan_destroy(sgm_plan *plan);

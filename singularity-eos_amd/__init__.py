@@ -117,6 +117,7 @@ _SIGS = {
     "sgm_plan_execute_bin": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                   C.POINTER(MixParams), PteStatus, vp]),
     "sgm_plan_describe": (ci, [vp, C.POINTER(C.c_long), ci]),
+    "sgm_plan_reorder": (ci, [vp, vp, vp]),
     "sgm_plan_destroy": (ci, [vp]),
     "sgm_mix_params_default": (None, [C.POINTER(MixParams)]),
     "sgm_mix_params_tight": (None, [C.POINTER(MixParams)]),
@@ -323,6 +324,9 @@ class Plan:
                                           ptr(lam),
                                           C.byref(params) if params is not None else None, st,
                                           stream)
+
+    def reorder(self, cost, stream=0):
+        return check(lib().sgm_plan_reorder(self.h, ptr(cost), stream), "plan_reorder")
 
     def describe(self):
         buf = (C.c_long * (6 * 16))()

@@ -101,7 +101,8 @@ hipError_t launch_pte_list_pt_ranged(const PteArgs &a, const MixParamsDev &P, co
                                      const long *range, long nmax, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
-                            unsigned long long *keys, int *ids, hipStream_t s);
+                            unsigned long long *keys, int *ids, hipStream_t s,
+                            const int *cost = nullptr, int threshold = 0);
 int device_cu_count();
 // Gruneisen::ComputeRhoMax (eos_gruneisen.hpp:231-329), host side
 Real gruneisen_compute_rho_max(Real s1, Real s2, Real s3, Real rho0);

@@ -88,7 +88,10 @@ hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hip
     return hipSuccess;
   }
   if ((e = hipEventRecord(p->ev[0], s)) != hipSuccess) return e;
-  for (size_t i = 0; i < nb; ++i) {
+  // most materials first: those bins carry the longest iteration caps
+  // (128 x nmat), so their slowest cells start early and overlap the rest
+  for (size_t ii = nb; ii-- > 0;) {
+    const size_t i = ii;
     if ((e = hipStreamWaitEvent(p->aux[i], p->ev[0], 0)) != hipSuccess) return e;
     if ((e = launch_one_bin(p, i, a, P, p->aux[i])) != hipSuccess) return e;
     if ((e = hipEventRecord(p->ev[i + 1], p->aux[i])) != hipSuccess) return e;
@@ -135,6 +138,40 @@ MixParamsDev params_dev(const sgm_mix_params *params) {
   return d;
 }
 
+// Sorts the plan's cells by key into d_sorted (radix sort of 64-bit keys);
+// keys_host receives the sorted keys when given. Synchronizes s.
+hipError_t sort_cells(sgm_plan *p, const int *cost, int threshold,
+                      std::vector<unsigned long long> *keys_host, hipStream_t s) {
+  const long ncell = p->ncell;
+  unsigned long long *d_keys = nullptr, *d_keys_sorted = nullptr;
+  int *d_ids = nullptr;
+  void *d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  hipError_t e = hipMalloc(&d_keys, sizeof(unsigned long long) * ncell);
+  if (e == hipSuccess) e = hipMalloc(&d_keys_sorted, sizeof(unsigned long long) * ncell);
+  if (e == hipSuccess) e = hipMalloc(&d_ids, sizeof(int) * ncell);
+  if (e == hipSuccess)
+    e = launch_plan_keys(p->d_offsets, p->d_mat, ncell, p->nnz, d_keys, d_ids, s, cost,
+                         threshold);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_ids,
+                                           p->d_sorted, (int)ncell, 0, 64, s);
+  if (e == hipSuccess) e = hipMalloc(&d_tmp, tmp_bytes + 16);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys_sorted, d_ids,
+                                           p->d_sorted, (int)ncell, 0, 64, s);
+  if (e == hipSuccess && keys_host) {
+    keys_host->resize(ncell);
+    e = hipMemcpyAsync(keys_host->data(), d_keys_sorted, sizeof(unsigned long long) * ncell,
+                       hipMemcpyDeviceToHost, s);
+  }
+  const hipError_t es = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = es;
+  for (void *q : {(void *)d_keys, (void *)d_keys_sorted, (void *)d_ids, d_tmp})
+    if (q) (void)hipFree(q);
+  return e;
+}
+
 } // namespace
 
 extern "C" {
@@ -155,35 +192,12 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
   bool ok = hipMalloc(&p->d_offsets, sizeof(int) * (ncell + 1)) == hipSuccess &&
             hipMalloc(&p->d_mat, nnz) == hipSuccess &&
             hipMalloc(&p->d_sorted, sizeof(int) * ncell) == hipSuccess;
-  unsigned long long *d_keys = nullptr, *d_keys_sorted = nullptr;
-  int *d_ids = nullptr;
-  void *d_tmp = nullptr;
-  size_t tmp_bytes = 0;
   std::vector<unsigned long long> keys;
   if (ok)
     ok = hipMemcpyAsync(p->d_offsets, offsets, sizeof(int) * (ncell + 1),
                         hipMemcpyDeviceToDevice, s) == hipSuccess &&
          hipMemcpyAsync(p->d_mat, mat_ids, nnz, hipMemcpyDeviceToDevice, s) == hipSuccess &&
-         hipMalloc(&d_keys, sizeof(unsigned long long) * ncell) == hipSuccess &&
-         hipMalloc(&d_keys_sorted, sizeof(unsigned long long) * ncell) == hipSuccess &&
-         hipMalloc(&d_ids, sizeof(int) * ncell) == hipSuccess;
-  if (ok) ok = launch_plan_keys(p->d_offsets, p->d_mat, ncell, nnz, d_keys, d_ids, s) == hipSuccess;
-  if (ok)
-    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_ids,
-                                            p->d_sorted, (int)ncell, 0, 64, s) == hipSuccess &&
-         hipMalloc(&d_tmp, tmp_bytes + 16) == hipSuccess &&
-         hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys_sorted, d_ids,
-                                            p->d_sorted, (int)ncell, 0, 64, s) == hipSuccess;
-  if (ok) {
-    keys.resize(ncell);
-    ok = hipMemcpyAsync(keys.data(), d_keys_sorted, sizeof(unsigned long long) * ncell,
-                        hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipStreamSynchronize(s) == hipSuccess;
-  }
-  if (d_keys) hipFree(d_keys);
-  if (d_keys_sorted) hipFree(d_keys_sorted);
-  if (d_ids) hipFree(d_ids);
-  if (d_tmp) hipFree(d_tmp);
+         sort_cells(p, nullptr, 0, &keys, s) == hipSuccess;
   if (!ok) {
     sgm_plan_destroy(p);
     return SGM_E_HIP;
@@ -250,6 +264,53 @@ int sgm_plan_describe(const sgm_plan *p, long *out, int max_bins) {
     ++k;
   }
   return (int)p->bins.size();
+}
+
+// Puts, within every material count, the cells whose cost (typically the
+// previous execution's Newton iteration counts) reaches 4x the mean cost
+// first: the slowest cells then start with the bulk instead of trailing it.
+// Results do not depend on the order.
+int sgm_plan_reorder(sgm_plan *p, const int *cost, sgm_stream_t stream) {
+  if (!p || !cost) return SGM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<int> h(p->ncell);
+  if (hipMemcpyAsync(h.data(), cost, sizeof(int) * p->ncell, hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return SGM_E_HIP;
+  double mean = 0;
+  for (int c : h) mean += c;
+  mean /= (double)p->ncell;
+  const int threshold = (int)std::max(1.0, 4.0 * mean);
+  std::vector<unsigned long long> keys;
+  if (sort_cells(p, cost, threshold, &keys, s) != hipSuccess) return SGM_E_HIP;
+  // Within each bin the costly cells now lead; spread them one per refill
+  // chunk (64 cells) so that each wave starts on one of them instead of a
+  // few waves holding them all. Bin bounds are unchanged: the material count
+  // still leads the key.
+  std::vector<int> ids(p->ncell), out(p->ncell);
+  if (hipMemcpy(ids.data(), p->d_sorted, sizeof(int) * p->ncell, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return SGM_E_HIP;
+  out = ids;
+  for (const Bin &b : p->bins) {
+    if (b.nm < 2 || b.n < 2) continue;
+    long nh = 0;
+    while (nh < b.n && !(keys[b.start + nh] & (1ull << 55))) ++nh;
+    if (nh == 0 || nh == b.n) continue;
+    const long stride = std::max<long>(1, std::min<long>(64, b.n / nh));
+    long h = 0, l = nh, o = b.start;
+    while (h < nh || l < b.n) {
+      if (h < nh) out[o++] = ids[b.start + h++];
+      for (long k = 1; k < stride && l < b.n; ++k) out[o++] = ids[b.start + l++];
+      if (h >= nh)
+        while (l < b.n) out[o++] = ids[b.start + l++];
+    }
+  }
+  if (hipMemcpy(p->d_sorted, out.data(), sizeof(int) * p->ncell, hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return SGM_E_HIP;
+  return SGM_OK;
 }
 
 // Times each candidate launch shape for every register-resident bin on

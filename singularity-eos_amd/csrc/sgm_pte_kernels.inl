@@ -164,6 +164,9 @@ __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], Driver
       const Real rb = s.rho[m] * s.vfrac[m];
       s.vfrac[m] = g;
       s.rho[m] = rb / g;
+      // the caller's pressure belongs to the caller's state: let the solver
+      // evaluate it at the guessed state (a NaN input pressure is recomputed)
+      s.press[m] = __builtin_nan("");
     }
     s.Tnorm = gT;
     if (ok) {

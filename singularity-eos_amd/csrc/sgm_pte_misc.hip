@@ -11,8 +11,12 @@ __global__ void zero_ints_kernel(int *p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// key = nmat << 56 | material ids in cell order (8 bits each; ids < 64, so bit
+// 55 is free). With a cost array, bit 55 is set on cells below the
+// threshold: within a material count, the costly cells sort first.
 __global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long ncell, long nnz,
-                                 unsigned long long *keys, int *ids) {
+                                 const int *cost, int threshold, unsigned long long *keys,
+                                 int *ids) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const long o0 = offsets[c], o1 = offsets[c + 1];
@@ -21,6 +25,7 @@ __global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long nc
     const int nm = (int)(o1 - o0);
     k = (unsigned long long)nm << 56;
     for (int m = 0; m < nm && m < 7; ++m) k |= (unsigned long long)mat[o0 + m] << (48 - 8 * m);
+    if (cost && cost[c] < threshold) k |= 1ull << 55;
   } else {
     k = 0xFFull << 56;
   }
@@ -37,10 +42,11 @@ hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
 }
 
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
-                            unsigned long long *keys, int *ids, hipStream_t s) {
+                            unsigned long long *keys, int *ids, hipStream_t s, const int *cost,
+                            int threshold) {
   if (ncell <= 0) return hipSuccess;
-  plan_keys_kernel<<<(int)((ncell + 255) / 256), 256, 0, s>>>(offsets, mat, ncell, nnz, keys,
-                                                                ids);
+  plan_keys_kernel<<<(int)((ncell + 255) / 256), 256, 0, s>>>(offsets, mat, ncell, nnz, cost,
+                                                                threshold, keys, ids);
   return hipGetLastError();
 }
 

@@ -257,3 +257,28 @@ def test_c2_full_size_invariants(sgm, workloads, pool, torch_gpu):
     T = b["temp"]
     Tc = T[off[:-1]]
     assert np.max(np.abs(T - Tc[cell])) == 0.0
+
+
+def test_reorder_by_cost_keeps_results(sgm, workloads, pool, torch_gpu):
+    """Ordering each bin by a previous run's iteration counts changes only the
+    schedule: every output is bit-identical before and after."""
+    torch = torch_gpu
+    ctx, _, _ = pool
+    a = workloads.uneven(ctx, 20000, seed=0x5EED0000 + 23)
+    params = sgm.mix_params(tight=True)
+    d = dev(torch, a)
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], 0)
+    outs = []
+    for step in range(2):
+        w = {k: d[k].clone() for k in FIELDS}
+        assert plan.execute(w["rho"], w["vfrac"], w["sie"], w["temp"], w["press"],
+                            d["vfrac_tot"], d["sie_tot"], params=params, conv=d["conv"],
+                            niter=d["niter"]) == 0
+        torch.cuda.synchronize()
+        outs.append({k: w[k].cpu().numpy() for k in FIELDS})
+        outs[-1]["niter"] = d["niter"].cpu().numpy().copy()
+        if step == 0:
+            plan.reorder(d["niter"])
+    for k in FIELDS + ("niter",):
+        assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
+    plan.close()
