@@ -81,6 +81,65 @@ def serve_bench(torch, ctx, a, params, nreq):
     return out
 
 
+def learned_bench(torch, ctx, params, ntrain, steps):
+    """Learned initial guess: train on the device on ntrain solved cells of
+    the uneven workload (another seed), evaluate on 2^16 held-out cells:
+    Newton iterations with and without the guess at the same 1e-10
+    tolerances, how often the guess was taken, and how many cells reached a
+    different equilibrium than from the solver's own start."""
+    wl = importlib.import_module("singularity-eos_amd.workloads")
+    L = importlib.import_module("singularity-eos_amd.learned")
+    t0 = time.perf_counter()
+    tr = wl.uneven(ctx, ntrain, seed=wl.SEED_BASE + 4)
+    held = wl.uneven(ctx, 1 << 16, seed=wl.SEED_BASE + 5)
+    mlp = L.MLP(seed=7)
+    X, tgt, rows = L.build_training_set(torch, mlp, ctx, tr, params)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = L.train(torch, mlp, X, tgt, rows, steps=steps, batch=8192)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    d = L.to_device(torch, held)
+    n, nnz = held["ncell"], held["nnz"]
+    base, c0, it0, _ = L.solve(torch, ctx, d, n, nnz, params)
+    gv, gT = mlp.guess(torch, ctx, d, n, nnz)
+    out, c1, it1, used = L.solve(torch, ctx, d, n, nnz, params, guess=(gv, gT))
+    # inference throughput (the guess kernel alone)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        mlp.guess(torch, ctx, d, n, nnz)
+    e1.record()
+    torch.cuda.synchronize()
+    inf_s = e0.elapsed_time(e1) * 1e-3 / 5
+    it0, it1 = it0.cpu().numpy(), it1.cpu().numpy()
+    c0, c1 = c0.cpu().numpy().astype(bool), c1.cpu().numpy().astype(bool)
+    used = used.cpu().numpy().astype(bool)
+    both = c0 & c1
+    off = held["offsets"][:-1]
+    T0 = base["temp"].cpu().numpy()[off]
+    T1 = out["temp"].cpu().numpy()[off]
+    diff = both & (np.abs(T1 - T0) > 1e-6 * np.abs(T0))
+    flops = 2 * (64 * 128 + 2 * 128 * 128 + 128 * 16)
+    res = {"train_cells": int(rows.numel()), "steps": steps, "batch": 8192,
+           "loss_first": hist[0][1], "loss_last": hist[-1][1],
+           "train_s": t2 - t1, "data_s": t1 - t0,
+           "held_out_cells": n, "guess_used_frac": float(used.mean()),
+           "converged_plain": float(c0.mean()), "converged_guided": float(c1.mean()),
+           "mean_iters_plain": float(it0[both].mean()), "mean_iters_guided": float(it1[both].mean()),
+           "p99_iters_plain": float(np.percentile(it0[both], 99)),
+           "p99_iters_guided": float(np.percentile(it1[both], 99)),
+           "different_equilibrium_frac": float(diff.sum() / max(1, both.sum())),
+           "inference_cells_per_s": n / inf_s,
+           "inference_mfma_TFLOPs": n * flops / inf_s / 1e12}
+    mlp.close()
+    log(f"[bench] learned guess: iterations {res['mean_iters_plain']:.2f} -> "
+        f"{res['mean_iters_guided']:.2f} (p99 {res['p99_iters_plain']:.0f} -> "
+        f"{res['p99_iters_guided']:.0f}), used {res['guess_used_frac']:.3f}, "
+        f"different equilibrium {res['different_equilibrium_frac']:.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +151,9 @@ def main():
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-train-cells", type=int, default=1 << 22)
+    ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--copy-cal", action="store_true",
                     help="end with one 1 GiB device copy (scale for FETCH/WRITE counters)")
@@ -260,6 +322,9 @@ def main():
 
     if not args.no_serve:
         extras["serving"] = serve_bench(torch, ctx, a, params, args.serve_requests)
+    if not args.no_learned:
+        extras["learned_guess"] = learned_bench(torch, ctx, params, args.learned_train_cells,
+                                                args.learned_steps)
 
     cpu = None
     flops_per_cell = None

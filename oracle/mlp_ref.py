@@ -66,7 +66,9 @@ def forward(P, X):
 
 
 def loss_grad(Y, T, wT=1.0):
-    """T: [B, 8] targets (alphas, standardized ln T, nmat). -> loss per row, dY"""
+    """T: [B, 8] targets (alphas, standardized ln T, nmat). Squared error of
+    the log volume fractions (log-softmax over the cell's materials, targets
+    floored at 1e-8) + wT (ln T error)^2. -> loss per row, dY (mean over B)"""
     B = len(Y)
     dY = np.zeros_like(Y)
     L = np.zeros(B)
@@ -75,11 +77,10 @@ def loss_grad(Y, T, wT=1.0):
         if nm < 2 or nm > 6:
             continue
         y = Y[r, :nm]
-        p = np.exp(y - y.max())
-        p /= p.sum()
-        e = p - T[r, :nm]
-        g = 2 * e
-        dY[r, :nm] = p * (g - (p * g).sum()) / B
+        lse = y.max() + np.log(np.exp(y - y.max()).sum())
+        p = np.exp(y - lse)
+        e = (y - lse) - np.log(np.maximum(T[r, :nm], 1e-8))
+        dY[r, :nm] = 2 * (e - p * e.sum()) / B
         et = Y[r, 6] - T[r, 6]
         dY[r, 6] = 2 * wT * et / B
         L[r] = (e * e).sum() + wT * et * et

@@ -173,8 +173,9 @@ __global__ void __launch_bounds__(256) fwd_kernel(const bf16 *__restrict__ wq,
   forward_tile(wq, P, X, act0, act1, wl, ylds, row0, nrow, &o);
 }
 
-// Loss: masked-softmax alphas vs target alphas (squared error, summed over
-// slots) + wT (standardized ln T - target)^2, mean over valid rows.
+// Loss: squared error of the log volume fractions (log-softmax of the slot
+// logits over the cell's materials, targets floored at 1e-8) + wT times the
+// squared error of the standardized ln T; mean over the batch.
 // targets row: [alpha0..alpha5, lnT_std, nmat (0 = row excluded)]
 __global__ void loss_kernel(const float *__restrict__ Y, const float *__restrict__ tgt,
                             const int *__restrict__ idx, long B, float wT, float inv_b,
@@ -194,18 +195,19 @@ __global__ void loss_kernel(const float *__restrict__ Y, const float *__restrict
   for (int s = 0; s < nm; ++s) mx = fmaxf(mx, y[s]);
   float p[SLOTS], sum = 0.f;
   for (int s = 0; s < nm; ++s) {
-    p[s] = __expf(y[s] - mx);
+    p[s] = expf(y[s] - mx);
     sum += p[s];
   }
-  float L = 0.f, g[SLOTS], pg = 0.f;
+  const float lse = mx + logf(sum);
+  float L = 0.f, e[SLOTS], esum = 0.f;
   for (int s = 0; s < nm; ++s) {
     p[s] /= sum;
-    const float e = p[s] - t[s];
-    L += e * e;
-    g[s] = 2.f * e;
-    pg += p[s] * g[s];
+    e[s] = (y[s] - lse) - logf(fmaxf(t[s], 1e-8f));
+    L += e[s] * e[s];
+    esum += e[s];
   }
-  for (int s = 0; s < nm; ++s) d[s] = inv_b * p[s] * (g[s] - pg);
+  // d/dy_j sum_s e_s^2 = 2 (e_j - p_j sum_s e_s)
+  for (int s = 0; s < nm; ++s) d[s] = inv_b * 2.f * (e[s] - p[s] * esum);
   const float et = y[6] - t[6];
   L += wT * et * et;
   d[6] = inv_b * 2.f * wT * et;

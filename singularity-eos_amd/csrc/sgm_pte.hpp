@@ -35,19 +35,28 @@ struct CellStatus {
 };
 
 // --- unrolled selects -------------------------------------------------------
+// A value the optimizer cannot look through. Selecting between two array
+// elements by a run-time index is otherwise rewritten into one load from a
+// run-time address, which pins the whole per-cell state in scratch memory
+// instead of registers; selecting between opaque values keeps it in VGPRs.
+SGM_HD Real opaque(Real x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(x));
+#endif
+  return x;
+}
+SGM_HD Real sel(bool c, Real a, Real b) { return c ? opaque(a) : opaque(b); }
 template <int N>
 SGM_HD Real pick(const Real (&a)[N], int i) {
-  Real r = a[0];
+  Real r = opaque(a[0]);
 #pragma unroll
-  for (int k = 1; k < N; ++k)
-    if (i == k) r = a[k];
+  for (int k = 1; k < N; ++k) r = sel(i == k, a[k], r);
   return r;
 }
 template <int N>
 SGM_HD void put(Real (&a)[N], int i, Real v) {
 #pragma unroll
-  for (int k = 0; k < N; ++k)
-    if (i == k) a[k] = v;
+  for (int k = 0; k < N; ++k) a[k] = sel(i == k, v, a[k]);
 }
 
 // --- small dense solve (mixed_cell_models.hpp:176-278) ----------------------
@@ -223,12 +232,27 @@ SGM_HD bool solve_small(Real (&a)[N * N], Real (&b)[N]) {
   return ok;
 }
 
+// The closure kernels stage the material registry into LDS (dynamic shared
+// memory) once per block; cells name their materials by registry index, so
+// every EOS parameter read is an LDS access.
+SGM_HD const EosDev &registry_entry(int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  extern __shared__ EosDev sgm_lds_registry[];
+  return sgm_lds_registry[i];
+#else
+  static const EosDev none{};  // the closure runs on the device only
+  (void)i;
+  return none;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Shared base state (PTESolverBase)
 // ---------------------------------------------------------------------------
 template <int NM, int NEQ>
 struct PteBase {
-  const EosDev *eos[NM];
+  unsigned char eidx[NM];  // registry index of each material
+  SGM_HD const EosDev &E(int m) const { return registry_entry(eidx[m]); }
   Lam lam[NM];
   Real rho[NM], vfrac[NM], sie[NM], press[NM], u[NM], rhobar[NM];
   Real jac[NEQ * NEQ], dx[NEQ], res[NEQ];
@@ -262,7 +286,7 @@ struct PteBase {
     Real vmax_sum = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      vmax[m] = smin(safety * robust::ratio(rhobar[m], eos_rho_pmin(*eos[m], T)), 1.0);
+      vmax[m] = smin(safety * robust::ratio(rhobar[m], eos_rho_pmin(E(m), T)), 1.0);
       if (vfrac[m] <= 0) vfrac[m] = vmax[m];
       vmax_sum += vmax[m];
     }
@@ -290,19 +314,19 @@ struct PteBase {
   }
   // :583-597
   SGM_HD Real PressureFromPreferred(int m, Real r, Real T, Real s) {
-    const EosDev &e = *eos[m];
+    const EosDev &e = E(m);
     if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
     return eos_P_rT(e, r, T, lam[m]);
   }
   // the same, with P(rho, T) already evaluated beside e (eos_e_rT_with_P)
   SGM_HD Real PressureFromPreferredP(int m, Real r, Real s, Real P_T) {
-    const EosDev &e = *eos[m];
+    const EosDev &e = E(m);
     if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
     return P_T;
   }
   // :599-620
   SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
-    const EosDev &e = *eos[m];
+    const EosDev &e = E(m);
     if (e.kind == KIND_IDEAL) {
       // IdealGas is the only model here whose preferred input includes
       // pressure: MinimumPressure() = 0, MaximumPressureAtTemperature() = 1e100
@@ -332,7 +356,7 @@ struct PteBase {
         Real dudt = 0, usum = 0;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-          const Real rho_max = eos_max_density(*eos[m]);
+          const Real rho_max = eos_max_density(E(m));
           Real s, cv;
           SieCvFromTAndPreferred(m, smin(rho[m], rho_max), press[m], Tguess, s, cv);
           usum += rhobar[m] * s;
@@ -342,7 +366,7 @@ struct PteBase {
         Tguess = smin(P.temperature_limit, smax(Tguess, Tn));
       }
 #pragma unroll
-      for (int m = 0; m < NM; ++m) Tguess = smax(eos_min_temperature(*eos[m]), Tguess);
+      for (int m = 0; m < NM; ++m) Tguess = smax(eos_min_temperature(E(m)), Tguess);
       bad = SetVfracFromT(Tguess);
       if (bad) {
         Tguess *= 10.0;
@@ -353,7 +377,7 @@ struct PteBase {
       bad = false;
 #pragma unroll
       for (int m = 0; m < NM; ++m) {
-        if (!bad && rho[m] < eos_rho_pmin(*eos[m], Tguess)) {
+        if (!bad && rho[m] < eos_rho_pmin(E(m), Tguess)) {
           bad = true;
           Tguess *= 10.0;
         }
@@ -373,8 +397,8 @@ struct PteBase {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       Real P_T;
-      sie[m] = eos_e_rT_with_P(*eos[m], rho[m], Tguess, lam[m], P_T);
-      if ((preferred_input(*eos[m]) & tq::pressure) && !bad_value(press[m])) {
+      sie[m] = eos_e_rT_with_P(E(m), rho[m], Tguess, lam[m], P_T);
+      if ((preferred_input(E(m)) & tq::pressure) && !bad_value(press[m])) {
         press[m] = robust::ratio(press[m], uscale);
       } else {
         press[m] = robust::ratio(PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
@@ -403,7 +427,7 @@ template <int NM>
 struct PteRhoT : PteBase<NM, NM> {
   using B = PteBase<NM, NM>;
   using B::dx;
-  using B::eos;
+  using B::E;
   using B::jac;
   using B::lam;
   using B::P;
@@ -432,7 +456,7 @@ struct PteRhoT : PteBase<NM, NM> {
     const Real pms = pick<NM>(press, ms);
 #pragma unroll
     for (int k = 1; k < NM; ++k) {
-      const Real pm = (k <= ms) ? press[k - 1] : press[k];
+      const Real pm = sel(k <= ms, press[k - 1], press[k]);
       res[k] = pms - pm;
     }
   }
@@ -503,11 +527,11 @@ struct PteRhoT : PteBase<NM, NM> {
       const Real vf_pert = vfrac[m] + dv;
       const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
       Real P_T;
-      Real e_pert = eos_e_rT_with_P(*eos[m], rho_pert, Tcur, lam[m], P_T);
+      Real e_pert = eos_e_rT_with_P(E(m), rho_pert, Tcur, lam[m], P_T);
       Real p_pert = robust::ratio(B::PressureFromPreferredP(m, rho_pert, e_pert, P_T), uscale);
       dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
       dedv[m] = robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dv);
-      e_pert = eos_e_rT_with_P(*eos[m], rho[m], Tpert, lam[m], P_T);
+      e_pert = eos_e_rT_with_P(E(m), rho[m], Tpert, lam[m], P_T);
       p_pert = robust::ratio(B::PressureFromPreferredP(m, rho[m], e_pert, P_T), uscale);
       dpdT[m] = robust::ratio((p_pert - press[m]), dT);
       dedT_sum += robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dT);
@@ -517,13 +541,13 @@ struct PteRhoT : PteBase<NM, NM> {
     jac[0] = dedT_sum;
 #pragma unroll
     for (int k = 1; k < NM; ++k) {
-      const Real dedv_k = (k <= ms) ? dedv[k - 1] : dedv[k];
+      const Real dedv_k = sel(k <= ms, dedv[k - 1], dedv[k]);
       jac[k] = dedv_k - dedv_s;
     }
 #pragma unroll
     for (int r = 1; r < NM; ++r) {
-      const Real dpdT_r = (r <= ms) ? dpdT[r - 1] : dpdT[r];
-      const Real dpdv_r = (r <= ms) ? dpdv[r - 1] : dpdv[r];
+      const Real dpdT_r = sel(r <= ms, dpdT[r - 1], dpdT[r]);
+      const Real dpdv_r = sel(r <= ms, dpdv[r - 1], dpdv[r]);
       jac[r * NM + 0] = dpdT_r - dpdT_s;
 #pragma unroll
       for (int c = 1; c < NM; ++c) jac[r * NM + c] = (c == r) ? dpdv_r + dpdv_s : dpdv_s;
@@ -539,7 +563,7 @@ struct PteRhoT : PteBase<NM, NM> {
     const Real dalphaskip = -s.result();
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const Real mydx = (m == ms) ? dalphaskip : ((m < ms) ? dx[m + 1] : dx[m]);
+      const Real mydx = sel(m == ms, dalphaskip, sel(m < ms, dx[m + 1], dx[m]));
       if (scale * mydx < -P.vfrac_safety_fac * vfrac[m])
         scale = smin(scale, fabs(P.vfrac_safety_fac * robust::ratio(vfrac[m], mydx)));
     }
@@ -548,7 +572,7 @@ struct PteRhoT : PteBase<NM, NM> {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       const Real rho_min =
-          smax(eos_rho_pmin(*eos[m], Tnorm * Tequil), eos_rho_pmin(*eos[m], Tnorm * Tnew));
+          smax(eos_rho_pmin(E(m), Tnorm * Tequil), eos_rho_pmin(E(m), Tnorm * Tnew));
       const Real alpha_max = smin(1.0, robust::ratio(rhobar[m], rho_min));
       if (!(alpha_max < vfrac[m])) {
         Real mydx;
@@ -577,7 +601,7 @@ struct PteRhoT : PteBase<NM, NM> {
     Tequil = Ttemp + scale * dx[0];
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const Real dxm = (m < ms) ? dx[m + 1] : dx[m];
+      const Real dxm = sel(m < ms, dx[m + 1], dx[m]);
       if (m != ms) vfrac[m] = vtemp[m] + scale * dxm;
     }
     NeumaierSum s;
@@ -590,7 +614,7 @@ struct PteRhoT : PteBase<NM, NM> {
     for (int m = 0; m < NM; ++m) {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      Real um = rhobar[m] * eos_e_rT_with_P(*eos[m], rho[m], Tc, lam[m], P_T);
+      Real um = rhobar[m] * eos_e_rT_with_P(E(m), rho[m], Tc, lam[m], P_T);
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
       press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
@@ -609,7 +633,7 @@ template <int NM>
 struct PtePT : PteBase<NM, 2> {
   using B = PteBase<NM, 2>;
   using B::dx;
-  using B::eos;
+  using B::E;
   using B::err;
   using B::jac;
   using B::lam;
@@ -650,7 +674,7 @@ struct PtePT : PteBase<NM, 2> {
     Tequil = 1;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      eos_rhoe_PT(*eos[m], Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      eos_rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
       vfrac[m] = robust::ratio(rhobar[m], rho[m]);
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
     }
@@ -672,12 +696,12 @@ struct PtePT : PteBase<NM, 2> {
     for (int m = 0; m < NM; ++m) {
       Real r_pert = rho[m];
       Real e_pert = sie[m];
-      eos_rhoe_PT(*eos[m], uscale * (Pequil + dp), Tnorm * Tequil, r_pert, e_pert, lam[m], err);
+      eos_rhoe_PT(E(m), uscale * (Pequil + dp), Tnorm * Tequil, r_pert, e_pert, lam[m], err);
       const Real drdp = robust::ratio(r_pert - rho[m], dp);
       const Real dudp = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dp);
       rbP += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdp;
       dudP += dudp;
-      eos_rhoe_PT(*eos[m], uscale * Pequil, Tnorm * (Tequil + dT), r_pert, e_pert, lam[m], err);
+      eos_rhoe_PT(E(m), uscale * Pequil, Tnorm * (Tequil + dT), r_pert, e_pert, lam[m], err);
       const Real drdT = robust::ratio(r_pert - rho[m], dT);
       const Real dudTm = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dT);
       rbT += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdT;
@@ -694,14 +718,14 @@ struct PtePT : PteBase<NM, 2> {
     if (scale * dx[0] < -0.95 * Tequil) scale = robust::ratio(-0.95 * Tequil, dx[0]);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const Real Pmin = eos_min_pressure(*eos[m]);
+      const Real Pmin = eos_min_pressure(E(m));
       scale = smin(fabs(scale),
                    fabs(0.95 * robust::ratio(robust::ratio(Pmin, uscale) - Pequil, dx[1])));
     }
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       const Real Ttest = (Tequil + scale * dx[0]) * Tnorm;
-      const Real Pmax = eos_max_pressure_at_T(*eos[m], Ttest, err);
+      const Real Pmax = eos_max_pressure_at_T(E(m), Ttest, err);
       scale = smin(fabs(scale),
                    fabs(0.95 * robust::ratio(robust::ratio(Pmax, uscale) - Pequil, dx[0])));
     }
@@ -719,7 +743,7 @@ struct PtePT : PteBase<NM, 2> {
     Pequil = Ptemp + scale * dx[1];
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      eos_rhoe_PT(*eos[m], Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      eos_rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
       vfrac[m] = robust::ratio(rhobar[m], rho[m]);
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
     }

@@ -41,7 +41,7 @@ __device__ __forceinline__ void load_cell(Sys &s, Real (&Tin)[NM], const EosDev 
 #pragma unroll
   for (int m = 0; m < NM; ++m) {
     const int k = off + m;
-    s.eos[m] = lds + a.mat_ids[k];
+    s.eidx[m] = a.mat_ids[k];
     s.rho[m] = a.rho[k];
     s.vfrac[m] = a.vfrac[k];
     s.press[m] = a.press[k];
@@ -145,7 +145,11 @@ __global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev 
 // Learned initial guess: restart the cell from the predicted volume
 // fractions (densities follow from the fixed partial densities) and
 // temperature (as Tnorm), and keep that start only if its initial residual is
-// below the solver's own start.
+// below the solver's own start and no condensed material (any model but the
+// ideal gas) sits more than a factor 2 from its density in the solver's own
+// start: condensed phases barely compress at these pressures, and a start far
+// from their reference density can lead Newton to a second, unphysical
+// equilibrium (an expanded Vinet branch with the gas collapsed).
 template <int NM, class Sys>
 __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], DriverState &d,
                                                  CellStatus &st, const EosDev *lds,
@@ -155,6 +159,9 @@ __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], Driver
   int used = 0;
   if (gT > 0) {  // NaN: no guess for this cell
     const Real err_plain = d.err;
+    Real rho_plain[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) rho_plain[m] = s.rho[m];
     load_cell<NM>(s, Tin, lds, a, P, c, off);
     bool ok = true;
 #pragma unroll
@@ -164,6 +171,8 @@ __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], Driver
       const Real rb = s.rho[m] * s.vfrac[m];
       s.vfrac[m] = g;
       s.rho[m] = rb / g;
+      if (s.E(m).kind != KIND_IDEAL)
+        ok = ok && (s.rho[m] <= 2 * rho_plain[m]) && (2 * s.rho[m] >= rho_plain[m]);
       // the caller's pressure belongs to the caller's state: let the solver
       // evaluate it at the guessed state (a NaN input pressure is recomputed)
       s.press[m] = __builtin_nan("");
@@ -187,8 +196,11 @@ __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], Driver
 // behind its slowest cell. Cells come from a per-wave reserve that the wave
 // refills `chunk` cells at a time with one atomic on the bin counter; the
 // sorted bin order keeps a wave's cells on the same material signature.
+// One wave per SIMD is allowed: the whole NM-material state then fits in the
+// 512 registers (VGPRs + AGPRs) of a lane instead of spilling to scratch.
 template <int NM, int METHOD>
-__global__ void __launch_bounds__(256) pte_bin_kernel(PteArgs a, MixParamsDev P,
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDev P,
                                                       const int *__restrict__ cells,
                                                       long ncells, int *counter, int chunk,
                                                       const long *range) {

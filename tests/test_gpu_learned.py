@@ -104,13 +104,24 @@ def test_training_reduces_newton_iterations(sgm, workloads, torch_gpu, L):
     assert c1.mean() >= c0.mean() - 0.002
     both = c0 & c1
     assert it1[both].mean() < it0[both].mean()
-    # equal accuracy: the two starts reach the same equilibrium
-    cell = np.repeat(np.arange(n), np.diff(held["offsets"]))
-    sel = both[cell]
-    for k in ("temp", "press", "vfrac"):
-        a0 = base[k].cpu().numpy()[sel]
-        a1 = out[k].cpu().numpy()[sel]
-        rel = np.abs(a1 - a0) / np.maximum(np.abs(a0), 1e-300)
-        assert np.percentile(rel, 99.9) < 1e-7, (k, np.percentile(rel, 99.9))
+    # equal accuracy: every cell solved from the learned start meets the
+    # closure's conservation and equilibrium bars, and (but for a tiny
+    # fraction of non-unique equilibria) it is the same equilibrium
+    off = held["offsets"]
+    cell = np.repeat(np.arange(n), np.diff(off))
+    r = {k: out[k].cpu().numpy() for k in ("rho", "vfrac", "sie", "temp", "press")}
+    rb_in = held["rho"] * held["vfrac"]
+    vs = np.bincount(cell, weights=r["vfrac"], minlength=n)
+    assert np.max(np.abs(vs - 1)[c1]) < 1e-12
+    assert np.max((np.abs(r["rho"] * r["vfrac"] - rb_in) / rb_in)[c1[cell]]) < 1e-12
+    u_in = held["sie_tot"] * np.bincount(cell, weights=rb_in, minlength=n)
+    u_out = np.bincount(cell, weights=r["rho"] * r["vfrac"] * r["sie"], minlength=n)
+    assert np.max((np.abs(u_out - u_in) / np.abs(u_in))[c1]) < 1e-9
+    Pm = np.bincount(cell, weights=r["press"] * r["vfrac"], minlength=n)
+    assert np.max((np.abs(r["press"] - Pm[cell]) / np.abs(Pm[cell]))[c1[cell]]) < 1e-9
+    T0 = base["temp"].cpu().numpy()[off[:-1]]
+    T1 = r["temp"][off[:-1]]
+    differ = both & (np.abs(T1 - T0) > 1e-6 * T0)
+    assert differ.mean() < 1e-3, differ.sum()
     mlp.close()
     ctx.close()
