@@ -186,13 +186,42 @@ int sgm_pte_PT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uin
                double *lambda, const sgm_mix_params *params, sgm_pte_status status,
                sgm_stream_t stream);
 
+/* Replaces PTESolverFixedT(nmat, eos, vfrac_tot, T_true, rho, vfrac, sie, temp,
+ * press, lambda, scratch, params) + PTESolver (mixed_cell_models.hpp:1451-1669,
+ * 2188-2291), as get_sg_eos_rho_t drives it (eos/get_sg_eos_rho_t.cpp:69-72),
+ * once per cell: the temperature T_true[c] is given; the volume fractions
+ * that equalize the material pressures at that temperature are solved for.
+ * Inputs rho, vfrac (guess), vfrac_tot[c], T_true[c]; outputs rho, vfrac, sie,
+ * press. temp is neither read nor written (the reference scales it by 1). */
+int sgm_pte_fixedT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                   const uint8_t *mat_ids, double *rho, double *vfrac, double *sie, double *temp,
+                   double *press, const double *vfrac_tot, const double *T_true, double *lambda,
+                   const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream);
+/* Replaces PTESolverFixedP(nmat, eos, vfrac_tot, P, rho, vfrac, sie, temp,
+ * press, lambda, scratch, params) + PTESolver (mixed_cell_models.hpp:1674-1921,
+ * 2188-2291), as get_sg_eos_rho_p drives it (eos/get_sg_eos_rho_p.cpp:67-70):
+ * the pressure P[c] is given; volume fractions and the common temperature are
+ * solved for. Inputs rho, vfrac (guess), temp (guess: max(default_tguess,
+ * temp) starts the temperature search), press (used by pressure-preferring
+ * EOS in that search), vfrac_tot[c], P[c]; outputs rho, vfrac, sie, temp,
+ * press. The reference reads two scale factors before assigning them in its
+ * temperature guess (mixed_cell_models.hpp:522-537, called from :1756); here
+ * both are zero there. Convergence compares the absolute pressure error with
+ * pte_rel_tolerance_p and pte_abs_tolerance_p, as the reference does. */
+int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                   const uint8_t *mat_ids, double *rho, double *vfrac, double *sie, double *temp,
+                   double *press, const double *vfrac_tot, const double *P, double *lambda,
+                   const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream);
+
 /* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
 /* No reference counterpart: a plan bins the cells of a fixed CSR topology by
  * material count (2..6 run register-resident templated kernels) and by
  * material signature, autotunes block shape and cells-per-wave per bin, and
  * captures one hipGraph whose persistent kernel nodes solve every bin.
- * method: 0 = RhoT, 1 = PT. Results are identical to sgm_pte_rhoT / sgm_pte_PT
- * on the same inputs. The plan keeps device copies of offsets and mat_ids. */
+ * method: 0 = RhoT, 1 = PT, 2 = FixedT, 3 = FixedP (the per-cell sie_tot
+ * array of execute then holds the cell temperature / pressure, and Tnorm is
+ * ignored). Results are identical to sgm_pte_rhoT / sgm_pte_PT /
+ * sgm_pte_fixedT / sgm_pte_fixedP on the same inputs. The plan keeps device copies of offsets and mat_ids. */
 int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                     const uint8_t *mat_ids, int method, sgm_plan **plan, sgm_stream_t stream);
 /* autotune (optional; uses the given arrays as scratch-copied input) */

@@ -288,12 +288,15 @@ void sgo_build_gruneisen_table(void *grv, double lRhoMin, double lRhoMax, int nR
 }
 
 // ---- PTE closure over a CSR cell list ------------------------------------
-// method 0 = PTESolverRhoT, 1 = PTESolverPT. offsets[ncell+1]; mat_eos[nnz]
-// indexes eos_handles; per-(cell,mat) arrays are in/out; lambda is nnz*3 or
-// null; Tnorm may be null (0 = no guess). Returns number of unconverged cells.
+// method 0 = PTESolverRhoT, 1 = PTESolverPT, 2 = PTESolverFixedT,
+// 3 = PTESolverFixedP. offsets[ncell+1]; mat_eos[nnz] indexes eos_handles;
+// per-(cell,mat) arrays are in/out; lambda is nnz*3 or null; Tnorm may be null
+// (0 = no guess; methods 0 and 1 only). cell_in[c] is the cell's specific
+// internal energy (methods 0, 1), its temperature (2) or its pressure (3).
+// Returns number of unconverged cells.
 long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eos,
                    void **eos_handles, int neos, double *rho, double *vfrac, double *sie,
-                   double *temp, double *press, const double *vfrac_tot, const double *sie_tot,
+                   double *temp, double *press, const double *vfrac_tot, const double *cell_in,
                    const double *Tnorm, const double *params_arr, double *lambda,
                    int *out_converged, int *out_niter, double *out_residual, int *out_err,
                    int nthreads) {
@@ -303,7 +306,7 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
   long nfail = 0;
 #pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : nfail)
   {
-    std::vector<Real> scratch(PTESolverRhoTRequiredScratch(8) + PTESolverPTRequiredScratch(8));
+    std::vector<Real> scratch(PTESolverFixedPRequiredScratch(8) + PTESolverRhoTRequiredScratch(8));
     EOS eos_cell[8];
 #pragma omp for schedule(dynamic, 256)
     for (long c = 0; c < ncell; ++c) {
@@ -314,12 +317,20 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
       const Real tn = Tnorm ? Tnorm[c] : 0.0;
       SolverStatus st;
       if (method == 0) {
-        PTERhoT s(nmat, eos_cell, vfrac_tot[c], sie_tot[c], rho + off, vfrac + off, sie + off,
+        PTERhoT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                   temp + off, press + off, lam, scratch.data(), tn, params);
         st = PTESolver(s);
-      } else {
-        PTEPT s(nmat, eos_cell, vfrac_tot[c], sie_tot[c], rho + off, vfrac + off, sie + off,
+      } else if (method == 1) {
+        PTEPT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                 temp + off, press + off, lam, scratch.data(), tn, params);
+        st = PTESolver(s);
+      } else if (method == 2) {
+        PTEFixedT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+                    temp + off, press + off, lam, scratch.data(), params);
+        st = PTESolver(s);
+      } else {
+        PTEFixedP s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+                    temp + off, press + off, lam, scratch.data(), params);
         st = PTESolver(s);
       }
       if (out_converged) out_converged[c] = st.converged ? 1 : 0;

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only. CPU restatement of the singularity-eos
 // pressure-temperature-equilibrium closure: MixParams, SolverStatus,
-// PTESolverBase, PTESolverRhoT, PTESolverPT, the PTESolver Newton driver,
+// PTESolverBase, PTESolverRhoT, PTESolverPT, PTESolverFixedT,
+// PTESolverFixedP, the PTESolver Newton driver,
 // ApproxTemperatureFromRhoMatU and MixUtils. Follows
 // singularity-eos/closure/mixed_cell_models.hpp line by line in algorithm;
 // the per-cell scratch layout mirrors the reference's AssignIncrement carving.
@@ -545,6 +546,262 @@ struct PTEPT : PTEBase {
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
       temp[m] = Tequil;
       press[m] = Pequil;
+    }
+    Residual();
+    return ResidualNorm();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverFixedT — mixed_cell_models.hpp:1451-1669. Temperature given;
+// unknowns the nmat volume fractions; residual: volume closure and nmat-1
+// neighbouring pressure differences.
+// ---------------------------------------------------------------------------
+inline int PTESolverFixedTRequiredScratch(int nmat) {
+  return nmat * nmat + 4 * nmat + 2 * nmat + 2 * nmat;  // :1451-1457
+}
+
+struct PTEFixedT : PTEBase {
+  Real *dpdv, *vtemp;
+  Real Tequil = 0, Ttemp = 0;
+  // :1496-1509: the base gets sie_tot = 1 and Tnorm = T_true, then Tnorm = 1
+  PTEFixedT(int nmat_, const EOS *eos_, Real vfrac_tot, Real T_true, Real *rho_, Real *vfrac_,
+            Real *sie_, Real *temp_, Real *press_, Real *lambda_, Real *scratch,
+            const MixParams &p)
+      : PTEBase(nmat_, nmat_, eos_, vfrac_tot, 1.0, rho_, vfrac_, sie_, temp_, press_, lambda_,
+                scratch, T_true, p) {
+    dpdv = inc(scratch, nmat);
+    vtemp = inc(scratch, nmat);
+    Tequil = T_true;
+    Ttemp = T_true;
+    Tnorm = 1.0;
+  }
+  // :1523-1551
+  Real Init() {
+    Tnorm = 1.0;
+    InitRhoBarandRho();
+    SetVfracFromT(Tequil);
+    uscale = 0.0;
+    for (int m = 0; m < nmat; m++) {
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      sie[m] = eos[m].e_rT(rho[m], Tequil, lam(m));
+      uscale += std::abs(sie[m]) * rho[m];
+      press[m] = eos[m].P_rT(rho[m], Tequil, lam(m));
+    }
+    uscale = std::max(uscale, 1.0e-14);
+    for (int m = 0; m < nmat; ++m) {
+      press[m] = robust::ratio(press[m], uscale);
+      u[m] = sie[m] * robust::ratio(rhobar[m], uscale);
+    }
+    Residual();
+    return ResidualNorm();
+  }
+  void Finalize() { FinalizeBase(); }
+  void Fixup() {}
+  // :1553-1560
+  void Residual() const {
+    const Real vsum = mix::sum_neumaier(vfrac, nmat);
+    residual[0] = vfrac_total - vsum;
+    for (int m = 0; m < nmat - 1; ++m) residual[1 + m] = press[m] - press[m + 1];
+  }
+  // :1562-1581
+  std::pair<bool, bool> CheckPTE() const {
+    Real mean_p = vfrac[0] * press[0];
+    Real error_p = 0;
+    for (int m = 1; m < nmat; ++m) {
+      mean_p += vfrac[m] * press[m];
+      error_p += residual[m] * residual[m];
+    }
+    error_p = std::sqrt(error_p);
+    const Real error_v = std::abs(residual[0]);
+    const bool converged_p = (error_p < params_.pte_rel_tolerance_p * std::abs(mean_p) ||
+                              error_p < params_.pte_abs_tolerance_p);
+    const bool converged_v = error_v < params_.pte_rel_tolerance_v;
+    const bool acceptable_p =
+        (error_p < params_.pte_rel_tolerance_p_sufficient * std::abs(mean_p) ||
+         error_p < params_.pte_abs_tolerance_p_sufficient);
+    const bool acceptable_v = error_v < params_.pte_rel_tolerance_v_sufficient;
+    return {converged_p && converged_v, acceptable_p && acceptable_v};
+  }
+  // :1583-1609
+  void Jacobian() {
+    for (int m = 0; m < nmat; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * params_.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      const Real p_pert = robust::ratio(eos[m].P_rT(rho_pert, Tequil, lam(m)), uscale);
+      dpdv[m] = std::min(robust::ratio((p_pert - press[m]), dv), -std::abs(params_.pte_max_dpdv));
+    }
+    for (int i = 0; i < neq * neq; ++i) jacobian[i] = 0.0;
+    for (int m = 0; m < nmat; ++m) jacobian[m] = 1.0;
+    for (int m = 0; m < nmat - 1; m++) {
+      jacobian[(m + 1) * neq + m] = -dpdv[m];
+      jacobian[(m + 1) * neq + m + 1] = dpdv[m + 1];
+    }
+  }
+  // :1611-1641
+  Real ScaleDx() {
+    Real scale = 1.0;
+    for (int m = 0; m < nmat; ++m) {
+      if (scale * dx[m] < -params_.vfrac_safety_fac * vfrac[m])
+        scale = robust::ratio(-params_.vfrac_safety_fac * vfrac[m], dx[m]);
+    }
+    for (int m = 0; m < nmat; m++) {
+      const Real rho_min = eos[m].RhoPmin(Tequil);
+      const Real alpha_max = robust::ratio(rhobar[m], rho_min);
+      if (alpha_max < vfrac[m]) continue;  // already unstable: no limit (as the reference)
+      if (scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+        scale = robust::ratio(0.5 * (alpha_max - vfrac[m]), dx[m]);
+    }
+    for (int i = 0; i < neq; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :1643-1664
+  Real TestUpdate(const Real scale, bool cache_state = false) {
+    if (cache_state)
+      for (int m = 0; m < nmat; ++m) vtemp[m] = vfrac[m];
+    for (int m = 0; m < nmat; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      u[m] = rhobar[m] * eos[m].e_rT(rho[m], Tequil, lam(m));
+      sie[m] = robust::ratio(u[m], rhobar[m]);
+      u[m] = robust::ratio(u[m], uscale);
+      press[m] = robust::ratio(eos[m].P_rT(rho[m], Tequil, lam(m)), uscale);
+    }
+    Residual();
+    return ResidualNorm();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverFixedP — mixed_cell_models.hpp:1674-1921. Pressure given;
+// unknowns the nmat volume fractions and the temperature; residual: each
+// material's pressure against P, then the volume closure.
+// ---------------------------------------------------------------------------
+inline int PTESolverFixedPRequiredScratch(int nmat) {
+  const int neq = nmat + 1;  // :1674-1680
+  return neq * neq + 4 * neq + 2 * nmat + 3 * nmat;
+}
+
+struct PTEFixedP : PTEBase {
+  Real *dpdv, *dpdT, *vtemp;
+  Real Tequil = 0, Ttemp = 0, Pequil = 0;
+  // :1721-1733: the base gets sie_tot = 1 and Tnorm = 0
+  PTEFixedP(int nmat_, const EOS *eos_, Real vfrac_tot, Real P, Real *rho_, Real *vfrac_,
+            Real *sie_, Real *temp_, Real *press_, Real *lambda_, Real *scratch,
+            const MixParams &p)
+      : PTEBase(nmat_, nmat_ + 1, eos_, vfrac_tot, 1.0, rho_, vfrac_, sie_, temp_, press_,
+                lambda_, scratch, 0.0, p) {
+    dpdv = inc(scratch, nmat);
+    dpdT = inc(scratch, nmat);
+    vtemp = inc(scratch, nmat);
+    Pequil = P;
+    // The reference's Init calls GetTguess, whose Newton step reads uscale and
+    // utotal_scale (:522-537) before anything has assigned them: their value
+    // there is indeterminate. This restatement fixes both at zero, which makes
+    // that step T - (sum rhobar e) / (sum rhobar cv).
+    uscale = 0.0;
+    utotal_scale = 0.0;
+  }
+  // :1747-1779
+  Real Init() {
+    InitRhoBarandRho();
+    const Real Tguess = GetTguess();
+    Tnorm = Tguess;
+    uscale = std::max(std::abs(Pequil), 1.0e-14);
+    for (int m = 0; m < nmat; m++) {
+      temp[m] = 1.0;
+      sie[m] = eos[m].e_rT(rho[m], Tguess, lam(m));
+    }
+    for (int m = 0; m < nmat; ++m) {
+      u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
+      press[m] = robust::ratio(eos[m].P_rT(rho[m], Tguess, lam(m)), uscale);
+    }
+    Residual();
+    Tequil = temp[0];
+    Ttemp = Tequil;
+    return ResidualNorm();
+  }
+  void Finalize() { FinalizeBase(); }
+  void Fixup() {}
+  // :1781-1788
+  void Residual() const {
+    const Real vsum = mix::sum_neumaier(vfrac, nmat);
+    for (int m = 0; m < nmat; ++m) residual[m] = robust::ratio(Pequil, uscale) - press[m];
+    residual[nmat] = vfrac_total - vsum;
+  }
+  // :1790-1808
+  std::pair<bool, bool> CheckPTE() const {
+    Real error_p = 0;
+    for (int m = 0; m < nmat; ++m) error_p += residual[m] * residual[m];
+    error_p = std::sqrt(error_p);
+    error_p *= uscale;
+    const Real error_v = std::abs(residual[neq - 1]);
+    const bool converged_p =
+        (error_p < params_.pte_rel_tolerance_p || error_p < params_.pte_abs_tolerance_p);
+    const bool converged_v = error_v < params_.pte_rel_tolerance_v;
+    const bool acceptable_p = (error_p < params_.pte_rel_tolerance_p_sufficient ||
+                               error_p < params_.pte_abs_tolerance_p_sufficient);
+    const bool acceptable_v = error_v < params_.pte_rel_tolerance_v_sufficient;
+    return {converged_p && converged_v, acceptable_p && acceptable_v};
+  }
+  // :1809-1850
+  void Jacobian() {
+    for (int m = 0; m < nmat; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * params_.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      Real p_pert = robust::ratio(
+          GetPressureFromPreferred(eos[m], rho_pert, Tnorm * Tequil, 0.0, lam(m), true), uscale);
+      dpdv[m] = std::min(robust::ratio((p_pert - press[m]), dv), -std::abs(params_.pte_max_dpdv));
+      const Real dT = Tequil * params_.derivative_eps;
+      p_pert = robust::ratio(
+          GetPressureFromPreferred(eos[m], rho[m], Tnorm * (Tequil + dT), 0.0, lam(m), true),
+          uscale);
+      dpdT[m] = robust::ratio((p_pert - press[m]), dT);
+    }
+    for (int i = 0; i < neq * neq; ++i) jacobian[i] = 0.0;
+    for (int m = 0; m < nmat; m++) {
+      jacobian[m * neq + m] = dpdv[m];
+      jacobian[m * neq + nmat] = dpdT[m];
+    }
+    for (int m = 0; m < neq - 1; ++m) jacobian[(neq - 1) * neq + m] = 1.0;
+  }
+  // :1851-1887
+  Real ScaleDx() {
+    Real scale = 1.0;
+    for (int m = 0; m < nmat; ++m) {
+      if (scale * dx[m] < -params_.vfrac_safety_fac * vfrac[m])
+        scale = -params_.vfrac_safety_fac * robust::ratio(vfrac[m], dx[m]);
+    }
+    const Real Tnew = Tequil + scale * dx[nmat];
+    for (int m = 0; m < nmat; m++) {
+      const Real rho_min =
+          std::max(eos[m].RhoPmin(Tnorm * Tequil), eos[m].RhoPmin(Tnorm * Tnew));
+      const Real alpha_max = robust::ratio(rhobar[m], rho_min);
+      if (alpha_max < vfrac[m]) continue;  // already unstable: no limit (as the reference)
+      if (scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+        scale = 0.5 * robust::ratio(alpha_max - vfrac[m], dx[m]);
+    }
+    if (scale * dx[nmat] < -0.95 * Tequil) scale = -0.95 * robust::ratio(Tequil, dx[nmat]);
+    for (int i = 0; i < neq; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :1889-1917
+  Real TestUpdate(const Real scale, bool cache_state = false) {
+    if (cache_state) {
+      Ttemp = Tequil;
+      for (int m = 0; m < nmat; ++m) vtemp[m] = vfrac[m];
+    }
+    Tequil = Ttemp + scale * dx[nmat];
+    for (int m = 0; m < nmat; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      u[m] = rhobar[m] * eos[m].e_rT(rho[m], Tnorm * Tequil, lam(m));
+      press[m] = robust::ratio(eos[m].P_rT(rho[m], Tnorm * Tequil, lam(m)), uscale);
+      sie[m] = robust::ratio(u[m], rhobar[m]);
+      u[m] = robust::ratio(u[m], uscale);
+      temp[m] = Tequil;
     }
     Residual();
     return ResidualNorm();

@@ -252,8 +252,8 @@ int sgm_set_IdealGas(sgm_ctx *ctx, int m, double gm1, double Cv) {
   if (!(Cv >= 0) || !(gm1 >= 0)) return SGM_E_ARG; // eos_ideal.hpp:66-71
   EosDev e{};
   e.kind = KIND_IDEAL;
-  e.p[0] = Cv;
-  e.p[1] = gm1;
+  e.m.p[0] = Cv;
+  e.m.p[1] = gm1;
   return set_common(ctx, m, e);
 }
 
@@ -261,10 +261,10 @@ int sgm_set_StiffGas(sgm_ctx *ctx, int m, double gm1, double Cv, double Pinf, do
   if (!(Cv >= 0) || !(gm1 >= 0)) return SGM_E_ARG; // eos_stiff.hpp:69-72
   EosDev e{};
   e.kind = KIND_STIFF;
-  e.p[0] = Cv;
-  e.p[1] = gm1;
-  e.p[2] = Pinf;
-  e.p[3] = qq;
+  e.m.p[0] = Cv;
+  e.m.p[1] = gm1;
+  e.m.p[2] = Pinf;
+  e.m.p[3] = qq;
   return set_common(ctx, m, e);
 }
 
@@ -274,7 +274,7 @@ int sgm_set_Gruneisen(sgm_ctx *ctx, int m, double C0, double s1, double s2, doub
   e.kind = KIND_GRUN;
   const double p[11] = {C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv,
                         rho_max > 0 ? rho_max : 0.99 * compute_rho_max(s1, s2, s3, rho0)};
-  for (int i = 0; i < 11; ++i) e.p[i] = p[i];
+  for (int i = 0; i < 11; ++i) e.m.p[i] = p[i];
   return set_common(ctx, m, e);
 }
 
@@ -312,13 +312,13 @@ int sgm_set_Vinet(sgm_ctx *ctx, int m, double rho0, double T0, double B0, double
   }
   EosDev e{};
   e.kind = KIND_VINET;
-  const double p[10] = {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP[1],
-                        pmath::cbrt(10 * robust::EPS()) * rho0};
-  for (int i = 0; i < 10; ++i) e.p[i] = p[i];
+  const double p[12] = {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP[1],
+                        pmath::cbrt(10 * robust::EPS()) * rho0, VIP[2], VIP[3]};
+  for (int i = 0; i < 12; ++i) e.m.p[i] = p[i];
   e.nvinet_skip = nskip;
-  e.coef = v->d_coef;
+  e.m.coef = v->d_coef;
   EosDev eh = e;
-  eh.coef = v->coef.data();
+  eh.m.coef = v->coef.data();
   {
     std::lock_guard<std::mutex> g(ctx->mu);
     if (ctx->vinets[m] && ctx->vinets[m]->d_coef) hipFree(ctx->vinets[m]->d_coef);
@@ -339,9 +339,9 @@ int sgm_set_TableRhoT(sgm_ctx *ctx, int m, const sgm_table_desc *desc) {
   if (!ctx->host_only && upload_table(*T) != hipSuccess) return SGM_E_HIP;
   EosDev e{};
   e.kind = KIND_TABLE;
-  e.tab = T->d_tab;
+  e.tab = T->ddev;
   EosDev eh = e;
-  eh.tab = &T->hdev;
+  eh.tab = T->hdev;
   std::lock_guard<std::mutex> g(ctx->mu);
   if (ctx->tables[m])
     for (void *p : ctx->tables[m]->d_arrays) hipFree(p);
@@ -477,8 +477,7 @@ static int pte_common(int method, sgm_ctx *ctx, long ncell, long nnz, const int 
   a.err = st.err;
   a.resid = st.residual;
   const MixParamsDev P = to_dev(params);
-  const hipError_t e = method == 0 ? launch_pte_plain_rhot(a, P, (hipStream_t)stream)
-                                   : launch_pte_plain_pt(a, P, (hipStream_t)stream);
+  const hipError_t e = launch_pte_plain(method, a, P, (hipStream_t)stream);
   return e == hipSuccess ? SGM_OK : SGM_E_HIP;
 }
 
@@ -498,6 +497,22 @@ int sgm_pte_PT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uin
                sgm_stream_t stream) {
   return pte_common(1, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press,
                     vfrac_tot, sie_tot, Tnorm, lambda, params, status, stream);
+}
+
+int sgm_pte_fixedT(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                   const uint8_t *mat_ids, double *rho, double *vfrac, double *sie, double *temp,
+                   double *press, const double *vfrac_tot, const double *T_true, double *lambda,
+                   const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream) {
+  return pte_common(PTE_FIXED_T, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp,
+                    press, vfrac_tot, T_true, nullptr, lambda, params, status, stream);
+}
+
+int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                   const uint8_t *mat_ids, double *rho, double *vfrac, double *sie, double *temp,
+                   double *press, const double *vfrac_tot, const double *P, double *lambda,
+                   const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream) {
+  return pte_common(PTE_FIXED_P, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp,
+                    press, vfrac_tot, P, nullptr, lambda, params, status, stream);
 }
 
 } // extern "C"

@@ -49,22 +49,28 @@ struct TableDev {
   const Real *rho_at_pmin;
 };
 
-// 256-byte POD describing one material. p[] holds the model parameters:
+// 256-byte POD describing one material; the closure and vector kernels stage
+// the registry into LDS, so everything an evaluation needs besides table and
+// coefficient arrays is one LDS read away. p[] holds the model parameters:
 //  IdealGas  {Cv, gm1}
 //  StiffGas  {Cv, gm1, Pinf, qq}
 //  Gruneisen {C0, s1, s2, s3, G0, b, rho0, T0, P0, Cv, rho_max}
-//  Vinet     {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP1, min_density}
-// Vinet Horner coefficients (d2..d40 then f0..f40) live in `coef`; nvinet is
-// the count of leading Horner steps that can be skipped because every
-// coefficient they touch is exactly zero (the sums stay exactly 0 there, so
-// skipping them leaves every bit of the result unchanged).
+//  Vinet     {rho0, T0, B0, BP0, A0, Cv0, E0, S0, VIP1, min_density, VIP2, VIP3}
+// Vinet Horner coefficients (d2..d40 then VIP0..VIP42) live in `coef`;
+// nvinet_skip counts the leading Horner steps that can be skipped because
+// every coefficient they touch is exactly zero (the sums stay exactly 0
+// there, so skipping them leaves every bit of the result unchanged). A table
+// material carries its whole TableDev record (grid, scalars, array pointers).
 struct alignas(16) EosDev {
   int kind;
   int nvinet_skip;
-  Real p[12];
-  const Real *coef;
-  const TableDev *tab;
-  Real pad[17];
+  union {
+    struct {
+      Real p[12];
+      const Real *coef;
+    } m;
+    TableDev tab;
+  };
 };
 static_assert(sizeof(EosDev) == 256, "EosDev is 256 bytes");
 
@@ -522,13 +528,14 @@ SGM_HD EP F_DT(const Real *p, const Real *coef, int nskip, Real rho, Real T) {
     sumB = d2tod40[ind - 2] * ind + onemx * sumB;
     sumE = VIP[2 + ind] + onemx * sumE;
   }
-  sumE = VIP[3] + onemx * sumE;
-  sumE = VIP[2] + onemx * sumE;
+  // VIP[3], VIP[2] are kept in p[11], p[10] (the registry entry, in LDS)
+  sumE = p[11] + onemx * sumE;
+  sumE = p[10] + onemx * sumE;
   sumP = 1.0 + sumP * (onemx * onemx);
   sumB = sumB * onemx;
   sumE = VIP1 * onemx * sumE;
   Real energy = 9.0 * robust::ratio(B0, VIP1 * VIP1 * rho0) *
-                    (VIP[2] - expetatimes1mx * (VIP[2] - sumE)) -
+                    (p[10] - expetatimes1mx * (p[10] - sumE)) -
                 (A0 * B0) * (robust::ratio(T0, rho0) - robust::ratio(T0, rho));
   Real pressure = 3.0 * B0 * x2inv * onemx * expetatimes1mx * sumP;
   EP out;
@@ -581,11 +588,11 @@ SGM_HD unsigned long preferred_input(const EosDev &e) {
 }
 SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::e_rT(e.p, rho, T);
-  case KIND_STIFF: return stiff::e_rT(e.p, rho, T);
-  case KIND_GRUN: return grun::e_rT(e.p, rho, T);
-  case KIND_VINET: return vinet::e_rT(e.p, e.coef, e.nvinet_skip, rho, T);
-  default: return table::e_rT(*e.tab, rho, T, lam);
+  case KIND_IDEAL: return ideal::e_rT(e.m.p, rho, T);
+  case KIND_STIFF: return stiff::e_rT(e.m.p, rho, T);
+  case KIND_GRUN: return grun::e_rT(e.m.p, rho, T);
+  case KIND_VINET: return vinet::e_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
+  default: return table::e_rT(e.tab, rho, T, lam);
   }
 }
 // e(rho, T) and, for models whose preferred input is (rho, T) (Vinet, table),
@@ -596,12 +603,12 @@ SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
   switch (e.kind) {
   case KIND_VINET: {
-    const vinet::EP o = vinet::F_DT<false>(e.p, e.coef, e.nvinet_skip, rho, T);
+    const vinet::EP o = vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
     P_T = o.P;
     return o.e;
   }
   case KIND_TABLE: {
-    const TableDev &t = *e.tab;
+    const TableDev &t = e.tab;
     const Real lRho = table::lRho_(t, rho), lT = table::lT_(t, T);
     lam.set(lRho, lT);
     const int w = table::loc_rho_T(t, lRho, lT);
@@ -613,65 +620,72 @@ SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P
 }
 SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::P_rT(e.p, rho, T);
-  case KIND_STIFF: return stiff::P_rT(e.p, rho, T);
-  case KIND_GRUN: return grun::P_rT(e.p, rho, T);
-  case KIND_VINET: return vinet::P_rT(e.p, e.coef, e.nvinet_skip, rho, T);
-  default: return table::P_rT(*e.tab, rho, T, lam);
+  case KIND_IDEAL: return ideal::P_rT(e.m.p, rho, T);
+  case KIND_STIFF: return stiff::P_rT(e.m.p, rho, T);
+  case KIND_GRUN: return grun::P_rT(e.m.p, rho, T);
+  case KIND_VINET: return vinet::P_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
+  default: return table::P_rT(e.tab, rho, T, lam);
   }
+}
+// e(rho, T) and P(rho, T) for every model: the shared evaluation where there
+// is one (Vinet, table), separate closed forms otherwise
+SGM_HD Real eos_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
+  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return eos_e_rT_with_P(e, rho, T, lam, P);
+  P = eos_P_rT(e, rho, T, lam);
+  return eos_e_rT(e, rho, T, lam);
 }
 SGM_HD Real eos_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::P_re(e.p, rho, sie);
-  case KIND_STIFF: return stiff::P_re(e.p, rho, sie);
-  case KIND_GRUN: return grun::P_re(e.p, rho, sie);
-  case KIND_VINET: return vinet::P_re(e.p, e.coef, e.nvinet_skip, rho, sie);
-  default: return table::P_re(*e.tab, rho, sie, lam);
+  case KIND_IDEAL: return ideal::P_re(e.m.p, rho, sie);
+  case KIND_STIFF: return stiff::P_re(e.m.p, rho, sie);
+  case KIND_GRUN: return grun::P_re(e.m.p, rho, sie);
+  case KIND_VINET: return vinet::P_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
+  default: return table::P_re(e.tab, rho, sie, lam);
   }
 }
 SGM_HD Real eos_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::T_re(e.p, rho, sie);
-  case KIND_STIFF: return stiff::T_re(e.p, rho, sie);
-  case KIND_GRUN: return grun::T_re(e.p, rho, sie);
-  case KIND_VINET: return vinet::T_re(e.p, e.coef, e.nvinet_skip, rho, sie);
-  default: return table::T_re(*e.tab, rho, sie, lam);
+  case KIND_IDEAL: return ideal::T_re(e.m.p, rho, sie);
+  case KIND_STIFF: return stiff::T_re(e.m.p, rho, sie);
+  case KIND_GRUN: return grun::T_re(e.m.p, rho, sie);
+  case KIND_VINET: return vinet::T_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
+  default: return table::T_re(e.tab, rho, sie, lam);
   }
 }
 SGM_HD Real eos_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL:
-  case KIND_STIFF: return e.p[0];
-  case KIND_GRUN: return e.p[9];
-  case KIND_VINET: return e.p[5];
-  default: return table::cv_rT(*e.tab, rho, T, lam);
+  case KIND_STIFF: return e.m.p[0];
+  case KIND_GRUN: return e.m.p[9];
+  case KIND_VINET: return e.m.p[5];
+  default: return table::cv_rT(e.tab, rho, T, lam);
   }
 }
 SGM_HD Real eos_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL:
-  case KIND_STIFF: return e.p[0];
-  case KIND_GRUN: return e.p[9];
-  case KIND_VINET: return e.p[5];
-  default: return table::cv_re(*e.tab, rho, sie, lam);
+  case KIND_STIFF: return e.m.p[0];
+  case KIND_GRUN: return e.m.p[9];
+  case KIND_VINET: return e.m.p[5];
+  default: return table::cv_re(e.tab, rho, sie, lam);
   }
 }
 SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::bmod_rT(e.p, rho, T);
-  case KIND_STIFF: return stiff::bmod_rT(e.p, rho, T);
-  case KIND_GRUN: return grun::bmod_rT(e.p, rho, T);
-  case KIND_VINET: return vinet::bmod_rT(e.p, e.coef, e.nvinet_skip, rho, T);
-  default: return table::bmod_rT(*e.tab, rho, T, lam);
+  case KIND_IDEAL: return ideal::bmod_rT(e.m.p, rho, T);
+  case KIND_STIFF: return stiff::bmod_rT(e.m.p, rho, T);
+  case KIND_GRUN: return grun::bmod_rT(e.m.p, rho, T);
+  case KIND_VINET: return vinet::bmod_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
+  default: return table::bmod_rT(e.tab, rho, T, lam);
   }
 }
 SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
-  case KIND_IDEAL: return ideal::bmod_re(e.p, rho, sie);
-  case KIND_STIFF: return stiff::bmod_re(e.p, rho, sie);
-  case KIND_GRUN: return grun::bmod_re(e.p, rho, sie);
-  case KIND_VINET: return vinet::bmod_re(e.p, e.coef, e.nvinet_skip, rho, sie);
-  default: return table::bmod_re(*e.tab, rho, sie, lam);
+  case KIND_IDEAL: return ideal::bmod_re(e.m.p, rho, sie);
+  case KIND_STIFF: return stiff::bmod_re(e.m.p, rho, sie);
+  case KIND_GRUN: return grun::bmod_re(e.m.p, rho, sie);
+  case KIND_VINET: return vinet::bmod_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
+  default: return table::bmod_re(e.tab, rho, sie, lam);
   }
 }
 // EosBase::MinInternalEnergyFromDensity = e(rho, 0) (eos_base.hpp:598-603);
@@ -680,41 +694,41 @@ SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
   Lam none;
   switch (e.kind) {
   case KIND_IDEAL: return 0.0;
-  case KIND_STIFF: return stiff::e_rT(e.p, rho, 0.);
-  case KIND_GRUN: return grun::e_rT(e.p, smin(rho, e.p[10]), 0.);
-  case KIND_VINET: return vinet::e_rT(e.p, e.coef, e.nvinet_skip, rho, 0.);
-  default: return lerp_rho(*e.tab, e.tab->sieCold, table::lRho_(*e.tab, rho));
+  case KIND_STIFF: return stiff::e_rT(e.m.p, rho, 0.);
+  case KIND_GRUN: return grun::e_rT(e.m.p, smin(rho, e.m.p[10]), 0.);
+  case KIND_VINET: return vinet::e_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, 0.);
+  default: return lerp_rho(e.tab, e.tab.sieCold, table::lRho_(e.tab, rho));
   }
   (void)none;
 }
 SGM_HD Real eos_min_density(const EosDev &e) {
-  if (e.kind == KIND_VINET) return e.p[9];
-  if (e.kind == KIND_TABLE) return table::rho_(*e.tab, e.tab->lRhoMin);
+  if (e.kind == KIND_VINET) return e.m.p[9];
+  if (e.kind == KIND_TABLE) return table::rho_(e.tab, e.tab.lRhoMin);
   return 0;
 }
 SGM_HD Real eos_min_temperature(const EosDev &e) {
-  if (e.kind == KIND_TABLE) return table::T_(*e.tab, e.tab->lTMin);
+  if (e.kind == KIND_TABLE) return table::T_(e.tab, e.tab.lTMin);
   return 0;
 }
 SGM_HD Real eos_max_density(const EosDev &e) {
-  if (e.kind == KIND_GRUN) return e.p[10];
-  if (e.kind == KIND_TABLE) return e.tab->rhoMax;
+  if (e.kind == KIND_GRUN) return e.m.p[10];
+  if (e.kind == KIND_TABLE) return e.tab.rhoMax;
   return 1e100;
 }
 SGM_HD Real eos_min_pressure(const EosDev &e) {
-  if (e.kind == KIND_GRUN) return grun::P_re(e.p, 0, 0);
+  if (e.kind == KIND_GRUN) return grun::P_re(e.m.p, 0, 0);
   if (e.kind == KIND_VINET) return -1e100;
-  if (e.kind == KIND_TABLE) return e.tab->PMin;
+  if (e.kind == KIND_TABLE) return e.tab.PMin;
   return 0;
 }
 // Gruneisen returns its maximum stable density here, as the reference does
 // (eos_gruneisen.hpp:160-163).
 SGM_HD Real eos_max_pressure_at_T(const EosDev &e, Real T, int &err) {
-  if (e.kind == KIND_GRUN) return grun::max_stable_density(e.p, T, err);
+  if (e.kind == KIND_GRUN) return grun::max_stable_density(e.m.p, T, err);
   return 1e100;
 }
 SGM_HD Real eos_rho_pmin(const EosDev &e, Real T) {
-  if (e.kind == KIND_TABLE) return table::rho_pmin(*e.tab, T);
+  if (e.kind == KIND_TABLE) return table::rho_pmin(e.tab, T);
   return 0.0;
 }
 // DensityEnergyFromPressureTemperature; rho is the in/out guess. The
@@ -722,15 +736,15 @@ SGM_HD Real eos_rho_pmin(const EosDev &e, Real T) {
 SGM_NOINL void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
                            int &err) {
   switch (e.kind) {
-  case KIND_IDEAL: ideal::rhoe_PT(e.p, P, T, rho, sie); return;
-  case KIND_STIFF: stiff::rhoe_PT(e.p, P, T, rho, sie); return;
-  case KIND_GRUN: grun::rhoe_PT(e.p, P, T, rho, sie, err); return;
-  case KIND_TABLE: table::rhoe_PT(*e.tab, P, T, rho, sie, lam); return;
+  case KIND_IDEAL: ideal::rhoe_PT(e.m.p, P, T, rho, sie); return;
+  case KIND_STIFF: stiff::rhoe_PT(e.m.p, P, T, rho, sie); return;
+  case KIND_GRUN: grun::rhoe_PT(e.m.p, P, T, rho, sie, err); return;
+  case KIND_TABLE: table::rhoe_PT(e.tab, P, T, rho, sie, lam); return;
   default: break;
   }
-  const Real rhomin = smax(0.0, e.p[9]);
+  const Real rhomin = smax(0.0, e.m.p[9]);
   const Real rhomax = 1e100;
-  auto f = [&](Real r) { return vinet::P_rT(e.p, e.coef, e.nvinet_skip, r, T); };
+  auto f = [&](Real r) { return vinet::P_rT(e.m.p, e.m.coef, e.nvinet_skip, r, T); };
   Real rhoguess = rho;
   if ((rhoguess <= rhomin) || (rhoguess >= rhomax)) {
     if ((rhomin < 12.0) && (12.0 < rhomax))
@@ -739,7 +753,7 @@ SGM_NOINL void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie
       rhoguess = 0.5 * (rhomin + rhomax);
   }
   root::find_root(f, P, rhoguess, rhomin, rhomax, robust::EPS(), robust::EPS(), rho);
-  sie = vinet::e_rT(e.p, e.coef, e.nvinet_skip, rho, T);
+  sie = vinet::e_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
 }
 
 // FillEos for the (rho, sie) -> {T, P, cv, bmod} and (rho, T) -> {sie, P, cv, bmod}
@@ -759,40 +773,40 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
         return;
       }
       if (st)
-        stiff::rhoe_PT(e.p, press, temp, rho, sie);
+        stiff::rhoe_PT(e.m.p, press, temp, rho, sie);
       else
-        ideal::rhoe_PT(e.p, press, temp, rho, sie);
+        ideal::rhoe_PT(e.m.p, press, temp, rho, sie);
     }
     if (output & pressure && output & specific_internal_energy) {
       if (output & density || output & temperature) {
         err |= ERRB_FILLEOS;
         return;
       }
-      sie = st ? stiff::e_rT(e.p, rho, temp) : ideal::e_rT(e.p, rho, temp);
+      sie = st ? stiff::e_rT(e.m.p, rho, temp) : ideal::e_rT(e.m.p, rho, temp);
     }
     if (output & temperature && output & specific_internal_energy) {
-      sie = st ? robust::ratio(press + (e.p[1] + 1.0) * e.p[2], e.p[1] * rho) + e.p[3]
-               : press / (e.p[1] * rho);
+      sie = st ? robust::ratio(press + (e.m.p[1] + 1.0) * e.m.p[2], e.m.p[1] * rho) + e.m.p[3]
+               : press / (e.m.p[1] * rho);
     }
-    if (output & pressure) press = st ? stiff::P_re(e.p, rho, sie) : ideal::P_re(e.p, rho, sie);
-    if (output & temperature) temp = st ? stiff::T_re(e.p, rho, sie) : ideal::T_re(e.p, rho, sie);
+    if (output & pressure) press = st ? stiff::P_re(e.m.p, rho, sie) : ideal::P_re(e.m.p, rho, sie);
+    if (output & temperature) temp = st ? stiff::T_re(e.m.p, rho, sie) : ideal::T_re(e.m.p, rho, sie);
     if (output & bulk_modulus)
-      bmod = st ? stiff::bmod_re(e.p, rho, sie) : ideal::bmod_re(e.p, rho, sie);
-    if (output & specific_heat) cv = e.p[0];
+      bmod = st ? stiff::bmod_re(e.m.p, rho, sie) : ideal::bmod_re(e.m.p, rho, sie);
+    if (output & specific_heat) cv = e.m.p[0];
     return;
   }
   case KIND_GRUN: {
     if (temperature & input && pressure & input) {
-      grun::rhoe_PT(e.p, press, temp, rho, sie, err);
+      grun::rhoe_PT(e.m.p, press, temp, rho, sie, err);
     } else if (density & output || specific_internal_energy & output) {
       err |= ERRB_FILLEOS;
       return;
     }
-    const Real r = smin(rho, e.p[10]);
-    if (output & pressure) press = grun::P_re(e.p, r, sie);
-    if (output & temperature) temp = grun::T_re(e.p, r, sie);
-    if (output & bulk_modulus) bmod = grun::bmod_re(e.p, r, sie);
-    if (output & specific_heat) cv = e.p[9];
+    const Real r = smin(rho, e.m.p[10]);
+    if (output & pressure) press = grun::P_re(e.m.p, r, sie);
+    if (output & temperature) temp = grun::T_re(e.m.p, r, sie);
+    if (output & bulk_modulus) bmod = grun::bmod_re(e.m.p, r, sie);
+    if (output & specific_heat) cv = e.m.p[9];
     return;
   }
   case KIND_VINET: {
@@ -804,13 +818,13 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
       return;
     }
     if (specific_internal_energy & input)
-      temp = vinet::T_re(e.p, e.coef, e.nvinet_skip, rho, sie);
+      temp = vinet::T_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
     const bool need_cs = (output & bulk_modulus) != 0;
-    vinet::EP o = need_cs ? vinet::F_DT<true>(e.p, e.coef, e.nvinet_skip, rho, temp)
-                          : vinet::F_DT<false>(e.p, e.coef, e.nvinet_skip, rho, temp);
+    vinet::EP o = need_cs ? vinet::F_DT<true>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp)
+                          : vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp);
     if (output & specific_internal_energy) sie = o.e;
     if (output & pressure) press = o.P;
-    if (output & specific_heat) cv = e.p[5];
+    if (output & specific_heat) cv = e.m.p[5];
     if (output & bulk_modulus) {
       const Real cs = sqrt(o.cs2);
       bmod = cs * cs * rho;
@@ -818,7 +832,7 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
     return;
   }
   default: {
-    const TableDev &t = *e.tab;
+    const TableDev &t = e.tab;
     Real lRho, lT;
     int w;
     if (output == 0) {

@@ -23,7 +23,8 @@ struct TableHost {
   std::vector<Real> PCold, sieCold, bModCold, dPdECold, dEdTCold, lTColdCrit;
   std::vector<Real> PMax, sielTMax, dEdTMax, gm1Max, rho_at_pmin;
   TableDev hdev{};       // pointers into the host vectors above
-  TableDev *d_tab = nullptr;  // device copy of the struct (pointers to device arrays)
+  TableDev ddev{};       // the same record with pointers to the device arrays
+  TableDev *d_tab = nullptr;  // device copy of ddev
   std::vector<void *> d_arrays;
 };
 
@@ -68,6 +69,8 @@ struct PteArgs {
   const int *offsets = nullptr;
   const uint8_t *mat_ids = nullptr;
   double *rho = nullptr, *vfrac = nullptr, *sie = nullptr, *temp = nullptr, *press = nullptr;
+  // per cell: vfrac_tot, and sie_tot = the specific internal energy (RhoT,
+  // PT), the temperature (FixedT) or the pressure (FixedP); Tnorm: RhoT, PT
   const double *vfrac_tot = nullptr, *sie_tot = nullptr, *Tnorm = nullptr;
   double *lambda = nullptr;
   int *conv = nullptr, *niter = nullptr, *err = nullptr;
@@ -76,29 +79,66 @@ struct PteArgs {
   const double *guess_vfrac = nullptr, *guess_T = nullptr;
   int *used_guess = nullptr;
 };
-hipError_t launch_pte_plain_rhot(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
-hipError_t launch_pte_plain_pt(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
-hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const int *cells,
-                                long n, hipStream_t s);
-hipError_t launch_pte_list_pt(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                              hipStream_t s);
-// batched bins (2..6 materials): list of cell ids with identical material count
-hipError_t launch_pte_bin_rhot(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
-                               long n, int *counter, int block, int cpw, int grid, hipStream_t s);
-hipError_t launch_pte_bin_pt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
-                             long n, int *counter, int block, int cpw, int grid, hipStream_t s);
-// the same kernels with the bin bounds read on the device from range = {start, count}
-// (cells into `cells`); nmax bounds count for the list kernel's grid
-hipError_t launch_pte_bin_rhot_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
-                                      const int *cells, const long *range, int *counter,
-                                      int block, int cpw, int grid, hipStream_t s);
-hipError_t launch_pte_bin_pt_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
-                                    const int *cells, const long *range, int *counter, int block,
-                                    int cpw, int grid, hipStream_t s);
-hipError_t launch_pte_list_rhot_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
-                                       const long *range, long nmax, hipStream_t s);
-hipError_t launch_pte_list_pt_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
-                                     const long *range, long nmax, hipStream_t s);
+// Closure formulations, by the method ids of the C ABI.
+enum PteMethod { PTE_RHOT = 0, PTE_PT = 1, PTE_FIXED_T = 2, PTE_FIXED_P = 3, PTE_NMETHODS = 4 };
+
+// Per-method launchers, one translation unit pair per method
+// (sgm_pte_plain_<name>.hip: plain + list kernels; sgm_pte_bins_<name>.hip:
+// the batched bin kernels for 2..6 materials). range: NULL, or device
+// {start, count} bounds into `cells` read by the kernel (n is then the host's
+// upper bound on count, used to size the list kernel's grid).
+#define SGM_PTE_LAUNCHERS(name)                                                                \
+  hipError_t launch_pte_plain_##name(const PteArgs &a, const MixParamsDev &P, hipStream_t s);  \
+  hipError_t launch_pte_list_##name(const PteArgs &a, const MixParamsDev &P, const int *cells, \
+                                    long n, const long *range, hipStream_t s);                 \
+  hipError_t launch_pte_bin_##name(int nm, const PteArgs &a, const MixParamsDev &P,            \
+                                   const int *cells, long n, const long *range, int *counter,  \
+                                   int block, int cpw, int grid, hipStream_t s);
+SGM_PTE_LAUNCHERS(rhot)
+SGM_PTE_LAUNCHERS(pt)
+SGM_PTE_LAUNCHERS(fixedt)
+SGM_PTE_LAUNCHERS(fixedp)
+#undef SGM_PTE_LAUNCHERS
+
+// the plain per-call kernel: one lane per cell, 1..8 materials
+inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParamsDev &P,
+                                   hipStream_t s) {
+  switch (method) {
+  case PTE_RHOT: return launch_pte_plain_rhot(a, P, s);
+  case PTE_PT: return launch_pte_plain_pt(a, P, s);
+  case PTE_FIXED_T: return launch_pte_plain_fixedt(a, P, s);
+  case PTE_FIXED_P: return launch_pte_plain_fixedp(a, P, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+// the plain solver over an explicit cell list
+inline hipError_t launch_pte_list(int method, const PteArgs &a, const MixParamsDev &P,
+                                  const int *cells, long n, hipStream_t s,
+                                  const long *range = nullptr) {
+  switch (method) {
+  case PTE_RHOT: return launch_pte_list_rhot(a, P, cells, n, range, s);
+  case PTE_PT: return launch_pte_list_pt(a, P, cells, n, range, s);
+  case PTE_FIXED_T: return launch_pte_list_fixedt(a, P, cells, n, range, s);
+  case PTE_FIXED_P: return launch_pte_list_fixedp(a, P, cells, n, range, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+// batched persistent bin (2..6 materials): cells all have nm materials
+inline hipError_t launch_pte_bin(int method, int nm, const PteArgs &a, const MixParamsDev &P,
+                                 const int *cells, long n, int *counter, int block, int cpw,
+                                 int grid, hipStream_t s, const long *range = nullptr) {
+  switch (method) {
+  case PTE_RHOT:
+    return launch_pte_bin_rhot(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+  case PTE_PT:
+    return launch_pte_bin_pt(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+  case PTE_FIXED_T:
+    return launch_pte_bin_fixedt(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+  case PTE_FIXED_P:
+    return launch_pte_bin_fixedp(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+  default: return hipErrorInvalidValue;
+  }
+}
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s,

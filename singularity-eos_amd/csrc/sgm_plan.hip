@@ -51,15 +51,20 @@ struct sgm_plan {
 
 namespace {
 
-int occupancy_grid(int method, int nm, int block) {
-  // persistent grid: enough waves to fill every CU at the kernel's occupancy
-  // limit; register use grows with the material count
+// Default persistent grids. The bin kernels run one wave per SIMD (their
+// whole per-cell state lives in registers), so one 256-thread block per CU
+// fills the GPU; every bin gets the whole GPU (measured: sharing the CUs
+// among the bins in proportion to their cells lengthens each bin more than
+// the overlap gains).
+void default_grids(std::vector<Bin> &bins) {
   const int ncu = device_cu_count();
-  const int waves_per_cu_guess = nm <= 2 ? 16 : nm <= 3 ? 12 : nm <= 4 ? 8 : 6;
-  const int waves_per_block = block / 64;
-  int g = ncu * std::max(1, waves_per_cu_guess / waves_per_block);
-  (void)method;
-  return g;
+  for (Bin &b : bins) {
+    if (b.nm < 2 || b.nm > 6) continue;
+    b.block = 256;
+    b.cpw = 64;
+    const long need = (b.n + b.block - 1) / b.block;  // no more blocks than cells fill
+    b.grid = (int)std::max<long>(1, std::min<long>(need, ncu));
+  }
 }
 
 hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixParamsDev &P,
@@ -67,12 +72,9 @@ hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixPara
   const Bin &b = p->bins[i];
   const int *cells = p->d_sorted + b.start;
   if (b.nm >= 2 && b.nm <= 6)
-    return p->method == 0 ? launch_pte_bin_rhot(b.nm, a, P, cells, b.n, p->d_counters + i,
-                                                b.block, b.cpw, b.grid, s)
-                          : launch_pte_bin_pt(b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
-                                              b.cpw, b.grid, s);
-  return p->method == 0 ? launch_pte_list_rhot(a, P, cells, b.n, s)
-                        : launch_pte_list_pt(a, P, cells, b.n, s);
+    return launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + i, b.block, b.cpw,
+                          b.grid, s);
+  return launch_pte_list(p->method, a, P, cells, b.n, s);
 }
 
 // Records the counter reset and every bin on stream s. With fork streams
@@ -179,7 +181,7 @@ extern "C" {
 int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                     const uint8_t *mat_ids, int method, sgm_plan **out, sgm_stream_t stream) {
   if (!ctx || !out || ncell < 1 || nnz < 1 || nnz > 2147483647L || ncell > 2147483647L ||
-      !offsets || !mat_ids || (method != 0 && method != 1))
+      !offsets || !mat_ids || method < 0 || method >= PTE_NMETHODS)
     return SGM_E_ARG;
   *out = nullptr;
   if (ctx->host_only) return SGM_E_NOGPU;
@@ -211,9 +213,6 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
     while (j < ncell && (int)(keys[j] >> 56) == nm) ++j;
     if (nm >= 2 && nm <= 6) {
       Bin b{nm, i, j - i};
-      b.block = 256;
-      b.cpw = 64;
-      b.grid = occupancy_grid(method, nm, b.block);
       p->bins.push_back(b);
     } else {
       // sorted keys put counts 1 before 2..6 and 7, 8, invalid after; keep
@@ -224,6 +223,7 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
     i = j;
   }
   (void)other;
+  default_grids(p->bins);
   if (hipMalloc(&p->d_counters, sizeof(int) * std::max<size_t>(1, p->bins.size())) !=
           hipSuccess ||
       hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -336,16 +336,19 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
   for (size_t bi = 0; bi < p->bins.size() && rc == SGM_OK; ++bi) {
     Bin &b = p->bins[bi];
     if (b.nm < 2 || b.nm > 6 || b.n < 64) continue;
+    // the bin's share of the GPU in waves (its default grid), scaled by
+    // 1/2, 1, 2; block shapes and refill chunks around the default
     const int blocks[3] = {64, 128, 256};
     const int cpws[3] = {64, 128, 256};
-    const int waves_per_cu[4] = {2, 4, 8, 16};
+    const int share_waves = b.grid * (b.block / 64);
+    const int scales[3] = {1, 2, 4};  // quarter-steps: x0.5, x1, x2
     float best = 1e30f;
     Bin bestb = b;
     for (int bl : blocks)
       for (int cpw : cpws)
-        for (int wpc : waves_per_cu) {
+        for (int sc : scales) {
           const int wpb = bl / 64;
-          const int grid = std::max(1, ncu * wpc / wpb);
+          const int grid = std::max(1, std::min(4 * ncu, share_waves * sc / 2 / wpb));
           float tot = 0;
           for (int r = 0; r < reps && rc == SGM_OK; ++r) {
             if (hipMemcpyAsync(w[0], rho, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
@@ -359,12 +362,8 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
             PteArgs a = make_args(p, w[0], w[1], w[4], w[2], w[3], vfrac_tot, sie_tot, nullptr,
                                   nullptr, sgm_pte_status{nullptr, nullptr, nullptr, nullptr});
             hipEventRecord(e0, s);
-            const hipError_t e =
-                p->method == 0
-                    ? launch_pte_bin_rhot(b.nm, a, P, p->d_sorted + b.start, b.n,
-                                          p->d_counters + bi, bl, cpw, grid, s)
-                    : launch_pte_bin_pt(b.nm, a, P, p->d_sorted + b.start, b.n,
-                                        p->d_counters + bi, bl, cpw, grid, s);
+            const hipError_t e = launch_pte_bin(p->method, b.nm, a, P, p->d_sorted + b.start,
+                                                b.n, p->d_counters + bi, bl, cpw, grid, s);
             hipEventRecord(e1, s);
             if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
               rc = SGM_E_HIP;
@@ -465,6 +464,9 @@ int sgm_plan_execute_guided(sgm_plan *p, double *rho, double *vfrac, double *sie
   if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot ||
       !guess_vfrac || !guess_T)
     return SGM_E_ARG;
+  // the guess replaces the volume fractions and Tnorm of the energy-closed
+  // formulations; the fixed-T / fixed-P ones have no temperature guess
+  if (p->method != PTE_RHOT && p->method != PTE_PT) return SGM_E_ARG;
   return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, nullptr, nullptr,
                         params, status, guess_vfrac, guess_T, used_guess, (hipStream_t)stream);
 }

@@ -2,7 +2,8 @@
 // cell per lane, with the material count NM fixed at compile time so every
 // per-material array lives in VGPRs. Restates
 // singularity-eos/closure/mixed_cell_models.hpp (PTESolverBase :329-786,
-// PTESolverRhoT :873-1193, PTESolverPT :1198-1446, PTESolver :2188-2291) in
+// PTESolverRhoT :873-1193, PTESolverPT :1198-1446, PTESolverFixedT
+// :1451-1669, PTESolverFixedP :1674-1921, PTESolver :2188-2291) in
 // the same operation order. Indices that the reference computes at run time
 // (the excluded material `ms`, column pivots) are resolved with unrolled
 // selects instead of dynamic register indexing.
@@ -235,14 +236,21 @@ SGM_HD bool solve_small(Real (&a)[N * N], Real (&b)[N]) {
 // The closure kernels stage the material registry into LDS (dynamic shared
 // memory) once per block; cells name their materials by registry index, so
 // every EOS parameter read is an LDS access.
+// Host compilations of this header (debugging tools that step the closure
+// code on the CPU) install a host-pointer registry here; the library runs the
+// closure on the device only and never sets it.
+inline const EosDev *&host_registry() {
+  static const EosDev *r = nullptr;
+  return r;
+}
 SGM_HD const EosDev &registry_entry(int i) {
 #if defined(__HIP_DEVICE_COMPILE__)
   extern __shared__ EosDev sgm_lds_registry[];
   return sgm_lds_registry[i];
 #else
-  static const EosDev none{};  // the closure runs on the device only
-  (void)i;
-  return none;
+  static const EosDev none{};
+  const EosDev *r = host_registry();
+  return r ? r[i] : none;
 #endif
 }
 
@@ -333,8 +341,8 @@ struct PteBase {
       // (eos_base.hpp:544-549), closed-form P-T inversion (eos_ideal.hpp:201-207)
       const Real Pmin = 0.0, Pmax = 1e100;
       const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
-      ideal::rhoe_PT(e.p, Pguess, T, r, s);
-      cv = e.p[0];
+      ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
+      cv = e.m.p[0];
     } else {
       s = eos_e_rT(e, r, T, lam[m]);
       cv = eos_cv_rT(e, r, T, lam[m]);
@@ -441,10 +449,16 @@ struct PteRhoT : PteBase<NM, NM> {
   using B::uscale;
   using B::utotal_scale;
   using B::vfrac;
+  static constexpr bool kStoresTemp = true;
   Real vtemp[NM];
   Real Tequil, Ttemp;
   int ms;
 
+  SGM_HD void SetCellInputs(Real vfrac_tot, Real sie_tot, Real Tnorm_in) {
+    B::vfrac_total = vfrac_tot;
+    B::sie_total = sie_tot;
+    Tnorm = Tnorm_in;
+  }
   // material index of reduced-system row/column k (1..NM-1)
   SGM_HD int mat_of(int k) const { return (k <= ms) ? k - 1 : k; }
 
@@ -649,8 +663,14 @@ struct PtePT : PteBase<NM, 2> {
   using B::utotal_scale;
   using B::vfrac;
   using B::vfrac_total;
+  static constexpr bool kStoresTemp = true;
   Real Pequil, Tequil, Ptemp, Ttemp;
 
+  SGM_HD void SetCellInputs(Real vfrac_tot, Real sie_tot, Real Tnorm_in) {
+    vfrac_total = vfrac_tot;
+    B::sie_total = sie_tot;
+    Tnorm = Tnorm_in;
+  }
   SGM_HD void Residual() {
     NeumaierSum sv, se;
 #pragma unroll
@@ -752,6 +772,314 @@ struct PtePT : PteBase<NM, 2> {
   }
   SGM_HD Real Tout() const { return Tequil * Tnorm; }
   SGM_HD Real Pout(int) const { return Pequil * uscale; }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverFixedT (:1451-1669). Temperature given; unknowns the volume
+// fractions; residual = volume closure + neighbouring pressure differences.
+// The solver never writes temperatures (its Finalize scales them by Tnorm=1).
+// ---------------------------------------------------------------------------
+template <int NM>
+struct PteFixedT : PteBase<NM, NM> {
+  using B = PteBase<NM, NM>;
+  using B::dx;
+  using B::E;
+  using B::jac;
+  using B::lam;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::vfrac;
+  using B::vfrac_total;
+  static constexpr bool kStoresTemp = false;
+  Real vtemp[NM];
+  Real Tequil;
+
+  // constructor (:1496-1509): base sie_tot = 1, Tnorm = 1 after the base call
+  SGM_HD void SetCellInputs(Real vfrac_tot, Real T_true, Real) {
+    vfrac_total = vfrac_tot;
+    B::sie_total = 1.0;
+    Tequil = T_true;
+    Tnorm = 1.0;
+  }
+  SGM_HD void Residual() {
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) s.add(vfrac[m]);
+    res[0] = vfrac_total - s.result();
+#pragma unroll
+    for (int m = 0; m < NM - 1; ++m) res[1 + m] = press[m] - press[m + 1];
+  }
+  // :1523-1551 (Tin unused: the temperature is the cell input)
+  SGM_HD Real Init(const Real (&)[NM]) {
+    Tnorm = 1.0;
+    B::InitRhoBarandRho();
+    B::SetVfracFromT(Tequil);
+    uscale = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      Real P_T;
+      sie[m] = eos_e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
+      uscale += fabs(sie[m]) * rho[m];
+      press[m] = P_T;
+    }
+    uscale = smax(uscale, 1.0e-14);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      press[m] = robust::ratio(press[m], uscale);
+      u[m] = sie[m] * robust::ratio(rhobar[m], uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD void Fixup() {}
+  // :1562-1581
+  SGM_HD void CheckPTE(bool &conv, bool &ok) const {
+    Real mean_p = vfrac[0] * press[0];
+    Real error_p = 0;
+#pragma unroll
+    for (int m = 1; m < NM; ++m) {
+      mean_p += vfrac[m] * press[m];
+      error_p += res[m] * res[m];
+    }
+    error_p = sqrt(error_p);
+    const Real error_v = fabs(res[0]);
+    const Real amp = fabs(mean_p);
+    conv = (error_p < P.pte_rel_tolerance_p * amp || error_p < P.pte_abs_tolerance_p) &&
+           (error_v < P.pte_rel_tolerance_v);
+    ok = (error_p < P.pte_rel_tolerance_p_sufficient * amp ||
+          error_p < P.pte_abs_tolerance_p_sufficient) &&
+         (error_v < P.pte_rel_tolerance_v_sufficient);
+  }
+  // :1583-1609
+  SGM_HD void Jacobian() {
+    Real dpdv[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      const Real p_pert = robust::ratio(eos_P_rT(E(m), rho_pert, Tequil, lam[m]), uscale);
+      dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
+    }
+#pragma unroll
+    for (int i = 0; i < NM * NM; ++i) jac[i] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) jac[m] = 1.0;
+#pragma unroll
+    for (int m = 0; m < NM - 1; m++) {
+      jac[(m + 1) * NM + m] = -dpdv[m];
+      jac[(m + 1) * NM + m + 1] = dpdv[m + 1];
+    }
+  }
+  // :1611-1641
+  SGM_HD Real ScaleDx() {
+    Real scale = 1.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      if (scale * dx[m] < -P.vfrac_safety_fac * vfrac[m])
+        scale = robust::ratio(-P.vfrac_safety_fac * vfrac[m], dx[m]);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real alpha_max = robust::ratio(rhobar[m], eos_rho_pmin(E(m), Tequil));
+      // already unstable (alpha_max < vfrac): no limit, as the reference
+      if (!(alpha_max < vfrac[m]) && scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+        scale = robust::ratio(0.5 * (alpha_max - vfrac[m]), dx[m]);
+    }
+#pragma unroll
+    for (int i = 0; i < NM; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :1643-1664
+  SGM_HD Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+#pragma unroll
+      for (int m = 0; m < NM; ++m) vtemp[m] = vfrac[m];
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      Real P_T;
+      const Real um = rhobar[m] * eos_e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
+      sie[m] = robust::ratio(um, rhobar[m]);
+      u[m] = robust::ratio(um, uscale);
+      press[m] = robust::ratio(P_T, uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD Real Tout() const { return Tequil; }
+  SGM_HD Real Pout(int m) const { return press[m] * uscale; }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverFixedP (:1674-1921). Pressure given; unknowns the volume fractions
+// and the temperature; residual = each material's pressure against P, then
+// the volume closure.
+// ---------------------------------------------------------------------------
+template <int NM>
+struct PteFixedP : PteBase<NM, NM + 1> {
+  using B = PteBase<NM, NM + 1>;
+  using B::dx;
+  using B::E;
+  using B::jac;
+  using B::lam;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::utotal_scale;
+  using B::vfrac;
+  using B::vfrac_total;
+  static constexpr bool kStoresTemp = true;
+  static constexpr int NEQ = NM + 1;
+  Real vtemp[NM];
+  Real Tequil, Ttemp, Pequil;
+
+  // constructor (:1721-1733): base sie_tot = 1, Tnorm = 0. The reference's
+  // GetTguess reads uscale and utotal_scale (:522-537) before Init assigns
+  // them, so their value there is indeterminate; both are zero here, which
+  // makes that Newton step T - (sum rhobar e) / (sum rhobar cv).
+  SGM_HD void SetCellInputs(Real vfrac_tot, Real P_in, Real) {
+    vfrac_total = vfrac_tot;
+    B::sie_total = 1.0;
+    Pequil = P_in;
+    Tnorm = 0.0;
+    uscale = 0.0;
+    utotal_scale = 0.0;
+  }
+  // the PreferredInput pressure with the energy looked up first
+  // (GetPressureFromPreferred(..., do_e_lookup = true), :583-597)
+  SGM_HD Real PressureLookup(int m, Real r, Real T) {
+    const EosDev &e = E(m);
+    if (preferred_input(e) & tq::specific_internal_energy)
+      return eos_P_re(e, r, eos_e_rT(e, r, T, lam[m]), lam[m]);
+    return eos_P_rT(e, r, T, lam[m]);
+  }
+  SGM_HD void Residual() {
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) s.add(vfrac[m]);
+    const Real pt = robust::ratio(Pequil, uscale);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) res[m] = pt - press[m];
+    res[NM] = vfrac_total - s.result();
+  }
+  // :1747-1779
+  SGM_HD Real Init(const Real (&Tin)[NM]) {
+    B::InitRhoBarandRho();
+    const Real Tguess = B::GetTguess(Tin);
+    Tnorm = Tguess;
+    uscale = smax(fabs(Pequil), 1.0e-14);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      Real P_T;
+      sie[m] = eos_e_and_P_rT(E(m), rho[m], Tguess, lam[m], P_T);
+      u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
+      press[m] = robust::ratio(P_T, uscale);
+    }
+    Residual();
+    Tequil = 1.0;  // temp[0], set to 1 above in the reference
+    Ttemp = Tequil;
+    return B::ResidualNorm();
+  }
+  SGM_HD void Fixup() {}
+  // :1790-1808
+  SGM_HD void CheckPTE(bool &conv, bool &ok) const {
+    Real error_p = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) error_p += res[m] * res[m];
+    error_p = sqrt(error_p);
+    error_p *= uscale;
+    const Real error_v = fabs(res[NM]);
+    conv = (error_p < P.pte_rel_tolerance_p || error_p < P.pte_abs_tolerance_p) &&
+           (error_v < P.pte_rel_tolerance_v);
+    ok = (error_p < P.pte_rel_tolerance_p_sufficient ||
+          error_p < P.pte_abs_tolerance_p_sufficient) &&
+         (error_v < P.pte_rel_tolerance_v_sufficient);
+  }
+  // :1809-1850
+  SGM_HD void Jacobian() {
+    Real dpdv[NM], dpdT[NM];
+    const Real dT = Tequil * P.derivative_eps;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      Real p_pert = robust::ratio(PressureLookup(m, rho_pert, Tnorm * Tequil), uscale);
+      dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
+      p_pert = robust::ratio(PressureLookup(m, rho[m], Tnorm * (Tequil + dT)), uscale);
+      dpdT[m] = robust::ratio((p_pert - press[m]), dT);
+    }
+#pragma unroll
+    for (int i = 0; i < NEQ * NEQ; ++i) jac[i] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      jac[m * NEQ + m] = dpdv[m];
+      jac[m * NEQ + NM] = dpdT[m];
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) jac[NM * NEQ + m] = 1.0;
+  }
+  // :1851-1887
+  SGM_HD Real ScaleDx() {
+    Real scale = 1.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      if (scale * dx[m] < -P.vfrac_safety_fac * vfrac[m])
+        scale = -P.vfrac_safety_fac * robust::ratio(vfrac[m], dx[m]);
+    const Real Tnew = Tequil + scale * dx[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real rho_min =
+          smax(eos_rho_pmin(E(m), Tnorm * Tequil), eos_rho_pmin(E(m), Tnorm * Tnew));
+      const Real alpha_max = robust::ratio(rhobar[m], rho_min);
+      if (!(alpha_max < vfrac[m]) && scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+        scale = 0.5 * robust::ratio(alpha_max - vfrac[m], dx[m]);
+    }
+    if (scale * dx[NM] < -0.95 * Tequil) scale = -0.95 * robust::ratio(Tequil, dx[NM]);
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :1889-1917
+  SGM_HD Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+      Ttemp = Tequil;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) vtemp[m] = vfrac[m];
+    }
+    Tequil = Ttemp + scale * dx[NM];
+    const Real Tc = Tnorm * Tequil;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      Real P_T;
+      const Real um = rhobar[m] * eos_e_and_P_rT(E(m), rho[m], Tc, lam[m], P_T);
+      press[m] = robust::ratio(P_T, uscale);
+      sie[m] = robust::ratio(um, rhobar[m]);
+      u[m] = robust::ratio(um, uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  SGM_HD Real Pout(int m) const { return press[m] * uscale; }
 };
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,9 @@
+// Batched persistent PTE bin kernels (2..6 materials), method PTESolverFixedT.
+#include "sgm_pte_kernels.inl"
+namespace sgm {
+hipError_t launch_pte_bin_fixedt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
+                             long n, const long *range, int *counter, int block, int cpw,
+                             int grid, hipStream_t s) {
+  return kern::bin_launch<2>(nm, a, P, cells, n, counter, block, cpw, grid, s, range);
+}
+} // namespace sgm

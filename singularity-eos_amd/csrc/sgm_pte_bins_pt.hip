@@ -2,12 +2,8 @@
 #include "sgm_pte_kernels.inl"
 namespace sgm {
 hipError_t launch_pte_bin_pt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
-                            long n, int *counter, int block, int cpw, int grid, hipStream_t s) {
-  return kern::bin_launch<1>(nm, a, P, cells, n, counter, block, cpw, grid, s);
-}
-hipError_t launch_pte_bin_pt_ranged(int nm, const PteArgs &a, const MixParamsDev &P,
-                                    const int *cells, const long *range, int *counter, int block,
-                                    int cpw, int grid, hipStream_t s) {
-  return kern::bin_launch<1>(nm, a, P, cells, 0, counter, block, cpw, grid, s, range);
+                             long n, const long *range, int *counter, int block, int cpw,
+                             int grid, hipStream_t s) {
+  return kern::bin_launch<1>(nm, a, P, cells, n, counter, block, cpw, grid, s, range);
 }
 } // namespace sgm

@@ -1,5 +1,6 @@
-// PTE closure kernels (templates; instantiated per method in the
-// sgm_pte_*.hip translation units so the build compiles them in parallel).
+// PTE closure kernels (templates; instantiated per method -- RhoT, PT,
+// FixedT, FixedP -- in the sgm_pte_*.hip translation units so the build
+// compiles them in parallel).
 //
 //  * pte_plain_kernel: the per-call path. One lane per cell over the caller's
 //    CSR order; the material count is read per cell and dispatched to the
@@ -31,8 +32,13 @@ __device__ __forceinline__ void stage_registry(EosDev *lds, const EosDev *reg, i
   __syncthreads();
 }
 
+// closure formulation by method id: 0 RhoT, 1 PT, 2 FixedT, 3 FixedP
 template <int NM, int METHOD>
-using SysOf = typename std::conditional<METHOD == 0, PteRhoT<NM>, PtePT<NM>>::type;
+using SysOf = typename std::conditional<
+    METHOD == 0, PteRhoT<NM>,
+    typename std::conditional<METHOD == 1, PtePT<NM>,
+                              typename std::conditional<METHOD == 2, PteFixedT<NM>,
+                                                        PteFixedP<NM>>::type>::type>::type;
 
 template <int NM, class Sys>
 __device__ __forceinline__ void load_cell(Sys &s, Real (&Tin)[NM], const EosDev *lds,
@@ -54,9 +60,7 @@ __device__ __forceinline__ void load_cell(Sys &s, Real (&Tin)[NM], const EosDev 
       s.lam[m].present = false;
     }
   }
-  s.vfrac_total = a.vfrac_tot[c];
-  s.sie_total = a.sie_tot[c];
-  s.Tnorm = a.Tnorm ? a.Tnorm[c] : 0.0;
+  s.SetCellInputs(a.vfrac_tot[c], a.sie_tot[c], a.Tnorm ? a.Tnorm[c] : 0.0);
   s.err = 0;
   s.P = P;
 }
@@ -71,7 +75,7 @@ __device__ __forceinline__ void store_cell(const Sys &s, const CellStatus &st, c
     a.rho[k] = s.rho[m];
     a.vfrac[k] = s.vfrac[m];
     a.sie[k] = s.sie[m];
-    a.temp[k] = Tout;
+    if (Sys::kStoresTemp) a.temp[k] = Tout;
     a.press[k] = s.Pout(m);
     if (a.lambda) {
       a.lambda[3L * k + 0] = s.lam[m].lRho;
@@ -190,10 +194,15 @@ __device__ __forceinline__ void start_from_guess(Sys &s, Real (&Tin)[NM], Driver
   if (a.used_guess) a.used_guess[c] = used;
 }
 
+// idle lanes that trigger a refill: refilling as soon as one lane is idle
+// measured faster on the uneven 2^20-cell workload than waiting for 16
+constexpr int REFILL_BATCH = 1;
+
 // Batched bin kernel with lane refill: every lane owns one cell and runs one
 // Newton iteration of it per trip of the loop; a lane whose cell has finished
-// stores it and takes the next cell of the bin at once, so a wave never idles
-// behind its slowest cell. Cells come from a per-wave reserve that the wave
+// stores it and, once REFILL_BATCH lanes of the wave are idle (or none is
+// busy), takes the next cell of the bin, so a wave does not idle behind its
+// slowest cell. Cells come from a per-wave reserve that the wave
 // refills `chunk` cells at a time with one atomic on the bin counter; the
 // sorted bin order keeps a wave's cells on the same material signature.
 // One wave per SIMD is allowed: the whole NM-material state then fits in the
@@ -222,9 +231,11 @@ __attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDe
   bool done = false;    // the bin is exhausted for this wave
   long rbase = 0, rend = 0;  // wave-uniform reserve [rbase, rend) of bin positions
   for (;;) {
-    // hand out cells to idle lanes
+    // hand out cells to idle lanes (REFILL_BATCH at a time)
     const unsigned long long need = __ballot(!active && !done);
-    if (need) {
+    const int nidle = __popcll(need);
+    const int nactive = __popcll(__ballot(active));
+    if (need && (nidle >= REFILL_BATCH || nactive == 0)) {
       const long cnt = __popcll(need);
       const long rank = __popcll(need & below);
       long j = -1;

@@ -5,11 +5,7 @@ hipError_t launch_pte_plain_pt(const PteArgs &a, const MixParamsDev &P, hipStrea
   return kern::plain_launch<1>(a, P, s);
 }
 hipError_t launch_pte_list_pt(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                             hipStream_t s) {
-  return kern::list_launch<1>(a, P, cells, n, s);
-}
-hipError_t launch_pte_list_pt_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
-                                     const long *range, long nmax, hipStream_t s) {
-  return kern::list_launch<1>(a, P, cells, nmax, s, range);
+                              const long *range, hipStream_t s) {
+  return kern::list_launch<1>(a, P, cells, n, s, range);
 }
 } // namespace sgm

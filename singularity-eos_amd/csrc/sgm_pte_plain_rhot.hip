@@ -5,11 +5,7 @@ hipError_t launch_pte_plain_rhot(const PteArgs &a, const MixParamsDev &P, hipStr
   return kern::plain_launch<0>(a, P, s);
 }
 hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                             hipStream_t s) {
-  return kern::list_launch<0>(a, P, cells, n, s);
-}
-hipError_t launch_pte_list_rhot_ranged(const PteArgs &a, const MixParamsDev &P, const int *cells,
-                                     const long *range, long nmax, hipStream_t s) {
-  return kern::list_launch<0>(a, P, cells, nmax, s, range);
+                              const long *range, hipStream_t s) {
+  return kern::list_launch<0>(a, P, cells, n, s, range);
 }
 } // namespace sgm

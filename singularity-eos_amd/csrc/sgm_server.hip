@@ -317,8 +317,7 @@ int launch_group(sgm_server *sv, Lane &ln, const Layout &L) {
     a.sie_tot = (const double *)D(L.st);
     a.conv = (int *)D(L.conv);
     if (sv->cfg.plain) {
-      ok(sv->cfg.method == 0 ? launch_pte_plain_rhot(a, sv->P, s)
-                             : launch_pte_plain_pt(a, sv->P, s));
+      ok(launch_pte_plain(sv->cfg.method, a, sv->P, s));
     } else {
       auto *keys = (unsigned long long *)D(L.keys), *keys2 = (unsigned long long *)D(L.keys2);
       int *ids = (int *)D(L.ids), *sorted = (int *)D(L.sorted);
@@ -334,16 +333,12 @@ int launch_group(sgm_server *sv, Lane &ln, const Layout &L) {
       const int grid = std::max(1, sv->ncu * 2);  // 8 waves per CU of 256-thread blocks
       for (int nm = 2; nm <= 6; ++nm) {
         const long *r = range + 2 * (nm - 1);
-        ok(sv->cfg.method == 0
-               ? launch_pte_bin_rhot_ranged(nm, a, sv->P, sorted, r, counters + nm, 256, 64, grid,
-                                            s)
-               : launch_pte_bin_pt_ranged(nm, a, sv->P, sorted, r, counters + nm, 256, 64, grid,
-                                          s));
+        ok(launch_pte_bin(sv->cfg.method, nm, a, sv->P, sorted, 0, counters + nm, 256, 64, grid,
+                          s, r));
       }
       for (int side = 0; side < 2; ++side) {
         const long *r = range + (side ? 12 : 0);
-        ok(sv->cfg.method == 0 ? launch_pte_list_rhot_ranged(a, sv->P, sorted, r, L.ncell, s)
-                               : launch_pte_list_pt_ranged(a, sv->P, sorted, r, L.ncell, s));
+        ok(launch_pte_list(sv->cfg.method, a, sv->P, sorted, L.ncell, s, r));
       }
     }
     if (e != hipSuccess) return SGM_E_HIP;

@@ -226,6 +226,7 @@ hipError_t upload_table(TableHost &T) {
   T.d_arrays.push_back(p);
   if ((e = hipMemcpy(p, &dd, sizeof(TableDev), hipMemcpyHostToDevice)) != hipSuccess) return e;
   T.d_tab = static_cast<TableDev *>(p);
+  T.ddev = dd;
   return hipSuccess;
 }
 

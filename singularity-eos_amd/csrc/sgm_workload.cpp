@@ -149,8 +149,8 @@ long sgm_workload_uneven(sgm_ctx *ctx, uint64_t seed, long n, const int *pool, i
         rr = r.lognormal(std::log(1e-3), 0.3);
       } else {
         const double ref = e.kind == KIND_STIFF   ? 1.0
-                           : e.kind == KIND_GRUN  ? e.p[6]
-                           : e.kind == KIND_VINET ? e.p[0]
+                           : e.kind == KIND_GRUN  ? e.m.p[6]
+                           : e.kind == KIND_VINET ? e.m.p[0]
                                                   : 2.7;
         rr = ref * r.uni(0.9, 1.1);
       }
