@@ -203,13 +203,20 @@ def main():
                           st, params=params, conv=conv, niter=niter, stream=stream)
         assert rc == 0, rc
 
+    step_ms = []
+
     def timed(k):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(k):
+        ev[0].record()
+        for i in range(k):
             step()
+            ev[i + 1].record()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / k
+        dt = (time.perf_counter() - t0) / k
+        step_ms[:] = [ev[i].elapsed_time(ev[i + 1]) for i in range(k)]
+        return dt
 
     for _ in range(args.warmup):
         step()
@@ -220,6 +227,8 @@ def main():
     mean_iter = float(niter.float().mean().item())
     log(f"[bench] batched: {dt * 1e3:.3f} ms/step, {cells_per_s:.4g} cells/s, "
         f"converged {conv_frac:.5f}, mean iters {mean_iter:.2f}")
+    log(f"[bench] batched step times (ms): " + " ".join(f"{x:.1f}" for x in step_ms))
+    step_stats = {"min": min(step_ms), "median": float(np.median(step_ms)), "max": max(step_ms)}
 
     # per-bin kernel timing (HIP events on the launch stream)
     per_bin = []
@@ -246,7 +255,8 @@ def main():
         nm = b["nmat"]
         per_bin.append(dict(nmat=nm, cells=cells, ms=ms, block=b["block"],
                             cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
-    extras = {"bins": per_bin, "converged_frac": conv_frac, "mean_newton_iters": mean_iter,
+    extras = {"bins": per_bin, "step_ms": step_stats, "converged_frac": conv_frac,
+              "mean_newton_iters": mean_iter,
               "max_newton_iters": int(niter.max().item())}
     if not args.no_reorder:
         # the plan reused step after step: each bin ordered by the previous

@@ -233,7 +233,17 @@ int sgm_plan_execute(sgm_plan *plan, double *rho, double *vfrac, double *sie, do
                      double *press, const double *vfrac_tot, const double *sie_tot,
                      const double *Tnorm, double *lambda, const sgm_mix_params *params,
                      sgm_pte_status status, sgm_stream_t stream);
-/* one bin's kernel without the graph (per-kernel timing and profiling) */
+/* No reference counterpart: sets one register-resident bin's launch shape
+ * (block 64/128/256 threads, cells claimed per wave >= 64, persistent grid),
+ * as sgm_plan_autotune would. Results do not change. */
+int sgm_plan_set_shape(sgm_plan *plan, int bin, int block, int cells_per_wave, int grid);
+/* No reference counterpart. Cells of the register-resident bins that are
+ * still iterating after `iterations` Newton iterations are parked (their
+ * solver state saved) and finished by a per-bin tail kernel whose waves hold
+ * only such cells, so the slowest cells neither stall the bulk of the bin nor
+ * hold its compute units. Results do not change. 0 disables; default 40. */
+int sgm_plan_set_straggler_threshold(sgm_plan *plan, int iterations);
+/* one bin's kernels without the graph (per-kernel timing and profiling) */
 int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, double *sie,
                          double *temp, double *press, const double *vfrac_tot,
                          const double *sie_tot, const double *Tnorm, double *lambda,

@@ -115,6 +115,8 @@ _SIGS = {
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_plan_create": (ci, [vp, cl, cl, vp, vp, ci, C.POINTER(vp), vp]),
+    "sgm_plan_set_straggler_threshold": (ci, [vp, ci]),
+    "sgm_plan_set_shape": (ci, [vp, ci, ci, ci, ci]),
     "sgm_plan_autotune": (ci, [vp, vp, vp, vp, vp, vp, vp, C.POINTER(MixParams), ci, vp]),
     "sgm_plan_execute": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(MixParams),
                               PteStatus, vp]),
@@ -340,6 +342,17 @@ class Plan:
 
     def reorder(self, cost, stream=0):
         return check(lib().sgm_plan_reorder(self.h, ptr(cost), stream), "plan_reorder")
+
+    def set_shape(self, bin, block, cells_per_wave, grid):
+        """Launch shape of one register-resident bin (as the autotuner sets it)."""
+        return check(lib().sgm_plan_set_shape(self.h, bin, block, cells_per_wave, grid),
+                     "plan_set_shape")
+
+    def set_straggler_threshold(self, iterations):
+        """Newton iterations after which a cell is parked for the tail kernel
+        (0 = never)."""
+        return check(lib().sgm_plan_set_straggler_threshold(self.h, int(iterations)),
+                     "plan_set_straggler_threshold")
 
     def describe(self):
         buf = (C.c_long * (6 * 16))()

@@ -79,8 +79,35 @@ struct PteArgs {
   const double *guess_vfrac = nullptr, *guess_T = nullptr;
   int *used_guess = nullptr;
 };
+// Straggler parking for the batched bin kernels: a cell still iterating after
+// `after` Newton iterations is saved (ParkSave, ParkWords(nm, nm + 1) FP64
+// words per record) and its lane takes new work; after the bin, a tail kernel
+// resumes the parked cells. Records are grouped by signature run (the
+// stretch of sorted bin positions whose cells have the same materials in the
+// same order), and each tail wave takes records of one run only, so a tail
+// wave's EOS branches agree across its lanes.
+struct ParkArgs {
+  int after = 0;                    // 0: never park
+  int stride = 0;                   // FP64 words per record
+  double *rec = nullptr;            // records
+  int *cell_of = nullptr;           // cell id per record
+  const int *run_of = nullptr;      // per bin position: signature run
+  const int *run_base = nullptr;    // per run: first record
+  const int *run_cap = nullptr;     // per run: records available
+  int *run_count = nullptr;         // per run: records claimed (claims past cap are not parked)
+  const int *wave_run = nullptr;    // per tail wave: its run
+  const int *wave_first = nullptr;  // per tail wave: its first record within the run
+  int nwaves = 0;
+};
+
 // Closure formulations, by the method ids of the C ABI.
 enum PteMethod { PTE_RHOT = 0, PTE_PT = 1, PTE_FIXED_T = 2, PTE_FIXED_P = 3, PTE_NMETHODS = 4 };
+
+// Lanes per parked cell in the tail kernels: with 4+ materials (the long
+// iteration caps) a group of 8 lanes runs each cell's line-search trials side
+// by side (pte_iterate_group); PT's trials are not independent, and small
+// cells have short caps, so those keep one lane per cell.
+constexpr int TailGroup(int method, int nm) { return (method != PTE_PT && nm >= 4) ? 8 : 1; }
 
 // Per-method launchers, one translation unit pair per method
 // (sgm_pte_plain_<name>.hip: plain + list kernels; sgm_pte_bins_<name>.hip:
@@ -93,7 +120,10 @@ enum PteMethod { PTE_RHOT = 0, PTE_PT = 1, PTE_FIXED_T = 2, PTE_FIXED_P = 3, PTE
                                     long n, const long *range, hipStream_t s);                 \
   hipError_t launch_pte_bin_##name(int nm, const PteArgs &a, const MixParamsDev &P,            \
                                    const int *cells, long n, const long *range, int *counter,  \
-                                   int block, int cpw, int grid, hipStream_t s);
+                                   int block, int cpw, int grid, const ParkArgs *park,         \
+                                   hipStream_t s);                                             \
+  hipError_t launch_pte_tail_##name(int nm, const PteArgs &a, const MixParamsDev &P,           \
+                                    const ParkArgs &park, hipStream_t s);
 SGM_PTE_LAUNCHERS(rhot)
 SGM_PTE_LAUNCHERS(pt)
 SGM_PTE_LAUNCHERS(fixedt)
@@ -126,16 +156,28 @@ inline hipError_t launch_pte_list(int method, const PteArgs &a, const MixParamsD
 // batched persistent bin (2..6 materials): cells all have nm materials
 inline hipError_t launch_pte_bin(int method, int nm, const PteArgs &a, const MixParamsDev &P,
                                  const int *cells, long n, int *counter, int block, int cpw,
-                                 int grid, hipStream_t s, const long *range = nullptr) {
+                                 int grid, hipStream_t s, const long *range = nullptr,
+                                 const ParkArgs *park = nullptr) {
   switch (method) {
   case PTE_RHOT:
-    return launch_pte_bin_rhot(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+    return launch_pte_bin_rhot(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
   case PTE_PT:
-    return launch_pte_bin_pt(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+    return launch_pte_bin_pt(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
   case PTE_FIXED_T:
-    return launch_pte_bin_fixedt(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+    return launch_pte_bin_fixedt(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
   case PTE_FIXED_P:
-    return launch_pte_bin_fixedp(nm, a, P, cells, n, range, counter, block, cpw, grid, s);
+    return launch_pte_bin_fixedp(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+// resumes a bin's parked cells (one 64-lane block per tail wave)
+inline hipError_t launch_pte_tail(int method, int nm, const PteArgs &a, const MixParamsDev &P,
+                                  const ParkArgs &park, hipStream_t s) {
+  switch (method) {
+  case PTE_RHOT: return launch_pte_tail_rhot(nm, a, P, park, s);
+  case PTE_PT: return launch_pte_tail_pt(nm, a, P, park, s);
+  case PTE_FIXED_T: return launch_pte_tail_fixedt(nm, a, P, park, s);
+  case PTE_FIXED_P: return launch_pte_tail_fixedp(nm, a, P, park, s);
   default: return hipErrorInvalidValue;
   }
 }

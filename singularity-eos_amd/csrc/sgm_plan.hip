@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <vector>
 
 using namespace sgm;
@@ -20,6 +21,8 @@ struct Bin {
   int block = 256;
   int cpw = 64;    // cells a wave claims per atomic on the bin counter (>= 64)
   int grid = 0;
+  ParkArgs park;   // straggler records (register-resident bins only)
+  int nruns = 0;   // signature runs in the bin
 };
 
 struct GraphKey {
@@ -37,6 +40,13 @@ struct sgm_plan {
   uint8_t *d_mat = nullptr;
   int *d_sorted = nullptr;  // cell ids grouped by bin
   int *d_counters = nullptr;
+  // straggler parking: per sorted position its signature run; per run its
+  // first record, capacity and claim counter; per tail wave its run and first
+  // record (all bins in one array each, bins at their own offsets)
+  int *d_run_of = nullptr, *d_run_base = nullptr, *d_run_cap = nullptr;
+  int *d_run_count = nullptr, *d_wave_run = nullptr, *d_wave_first = nullptr;
+  int nruns = 0;
+  int park_after = 40;           // Newton iterations before a cell is parked; 0 = never
   std::vector<Bin> bins;
   hipStream_t cap_stream = nullptr;
   // capture-time fork streams: each bin's kernel is its own graph branch, so
@@ -51,55 +61,174 @@ struct sgm_plan {
 
 namespace {
 
-// Default persistent grids. The bin kernels run one wave per SIMD (their
-// whole per-cell state lives in registers), so one 256-thread block per CU
-// fills the GPU; every bin gets the whole GPU (measured: sharing the CUs
-// among the bins in proportion to their cells lengthens each bin more than
-// the overlap gains).
+// Default persistent grids. The bin and tail kernels run two waves per SIMD,
+// so 8 waves per CU fill the GPU. Each bin gets 4 waves per CU: the bins'
+// bulk phases overlap, and SIMDs stay free for the tail kernels, which carry
+// the longest chains (measured on the uneven 2^20-cell workload). One-wave
+// blocks: a SIMD whose wave has finished takes a wave of any bin at once,
+// instead of waiting for the other waves of a multi-wave block.
 void default_grids(std::vector<Bin> &bins) {
   const int ncu = device_cu_count();
   for (Bin &b : bins) {
     if (b.nm < 2 || b.nm > 6) continue;
-    b.block = 256;
+    b.block = 64;
     b.cpw = 64;
-    const long need = (b.n + b.block - 1) / b.block;  // no more blocks than cells fill
-    b.grid = (int)std::max<long>(1, std::min<long>(need, ncu));
+    const long need = (b.n + b.block - 1) / b.block;  // no more waves than cells fill
+    b.grid = (int)std::max<long>(1, std::min<long>(need, 4L * ncu));
   }
 }
 
+// Straggler records for every register-resident bin, from the sort keys of
+// the bin's positions: the cells of one signature (a "run"; contiguous in
+// signature order, scattered after a cost reorder) of count L get
+// min(L, max(64, L/4 rounded up to 64)) records of ParkWords FP64 words and
+// one tail wave per 64 / TailGroup records. (About 1-3 % of the cells of the
+// uneven workload run past 40 iterations; a run whose records are used up
+// keeps its further stragglers in the bulk kernel.)
+hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) {
+  std::vector<int> run_of(p->ncell, 0), run_base, run_cap, wave_run, wave_first;
+  std::vector<size_t> run0(p->bins.size()), wave0(p->bins.size());
+  std::vector<long> nrec(p->bins.size(), 0);
+  for (size_t bi = 0; bi < p->bins.size(); ++bi) {
+    Bin &b = p->bins[bi];
+    run0[bi] = run_base.size();
+    wave0[bi] = wave_run.size();
+    if (b.nm < 2 || b.nm > 6) continue;
+    std::map<unsigned long long, int> run_id;
+    std::vector<long> len;
+    for (long j = 0; j < b.n; ++j) {
+      const unsigned long long k = keys[b.start + j] & ~(1ull << 55);  // signature only
+      auto it = run_id.find(k);
+      int r;
+      if (it == run_id.end()) {
+        r = (int)len.size();
+        run_id.emplace(k, r);
+        len.push_back(0);
+      } else {
+        r = it->second;
+      }
+      run_of[b.start + j] = r;
+      ++len[r];
+    }
+    for (int r = 0; r < (int)len.size(); ++r) {
+      const long cap = std::min<long>(len[r], std::max<long>(64, (len[r] / 4 + 63) / 64 * 64));
+      run_base.push_back((int)nrec[bi]);
+      run_cap.push_back((int)cap);
+      const int per_wave = 64 / TailGroup(p->method, b.nm);
+      for (long k = 0; k < cap; k += per_wave) {
+        wave_run.push_back(r);
+        wave_first.push_back((int)k);
+      }
+      nrec[bi] += cap;
+    }
+    b.nruns = (int)len.size();
+  }
+  p->nruns = (int)run_base.size();
+  auto up = [](int **d, const std::vector<int> &h) {
+    if (h.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(d, sizeof(int) * h.size());
+    if (e == hipSuccess) e = hipMemcpy(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = up(&p->d_run_of, run_of);
+  if (e == hipSuccess) e = up(&p->d_run_base, run_base);
+  if (e == hipSuccess) e = up(&p->d_run_cap, run_cap);
+  if (e == hipSuccess) e = up(&p->d_run_count, run_cap);  // contents reset per execution
+  if (e == hipSuccess) e = up(&p->d_wave_run, wave_run);
+  if (e == hipSuccess) e = up(&p->d_wave_first, wave_first);
+  for (size_t bi = 0; bi < p->bins.size() && e == hipSuccess; ++bi) {
+    Bin &b = p->bins[bi];
+    if (b.nm < 2 || b.nm > 6 || nrec[bi] == 0) continue;
+    ParkArgs &k = b.park;
+    k.stride = ParkWords(b.nm, b.nm + 1);
+    e = hipMalloc(&k.rec, sizeof(double) * k.stride * (size_t)nrec[bi]);
+    if (e == hipSuccess) e = hipMalloc(&k.cell_of, sizeof(int) * nrec[bi]);
+    k.run_of = p->d_run_of + b.start;
+    k.run_base = p->d_run_base + run0[bi];
+    k.run_cap = p->d_run_cap + run0[bi];
+    k.run_count = p->d_run_count + run0[bi];
+    k.wave_run = p->d_wave_run + wave0[bi];
+    k.wave_first = p->d_wave_first + wave0[bi];
+    k.nwaves = (int)((bi + 1 < p->bins.size() ? wave0[bi + 1] : wave_run.size()) - wave0[bi]);
+  }
+  return e;
+}
+
+void free_park(sgm_plan *p) {
+  for (Bin &b : p->bins)
+    for (void *q : {(void *)b.park.rec, (void *)b.park.cell_of})
+      if (q) (void)hipFree(q);
+  for (int *q : {p->d_run_of, p->d_run_base, p->d_run_cap, p->d_run_count, p->d_wave_run,
+                 p->d_wave_first})
+    if (q) (void)hipFree(q);
+}
+
+// Resets the bin counters and the straggler claim counters (fill kernels, so
+// the resets can be captured).
+hipError_t reset_counters(sgm_plan *p, hipStream_t s) {
+  hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
+  if (e == hipSuccess && p->park_after > 0 && p->nruns > 0)
+    e = launch_zero_ints(p->d_run_count, p->nruns, s);
+  return e;
+}
+
+// One bin: the persistent kernel, then (with parking) the in-order listing of
+// its parked cells and the tail kernel that finishes them.
 hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixParamsDev &P,
                           hipStream_t s) {
   const Bin &b = p->bins[i];
   const int *cells = p->d_sorted + b.start;
-  if (b.nm >= 2 && b.nm <= 6)
-    return launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + i, b.block, b.cpw,
-                          b.grid, s);
-  return launch_pte_list(p->method, a, P, cells, b.n, s);
+  if (b.nm < 2 || b.nm > 6) return launch_pte_list(p->method, a, P, cells, b.n, s);
+  ParkArgs pk = b.park;
+  pk.after = pk.rec ? p->park_after : 0;
+  hipError_t e = launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
+                                b.cpw, b.grid, s, nullptr, &pk);
+  if (e != hipSuccess || pk.after <= 0) return e;
+  return launch_pte_tail(p->method, b.nm, a, P, pk, s);
+}
+
+// Graph branches, at most four (the process has four hardware queues; a
+// fifth branch would share a queue with another and wait for all of it). The
+// bins of 6, 5, 4 and 3 materials each get a branch; the remaining bins follow
+// the 6-material bin, whose bulk phase ends first (its cells' long chains run
+// in its tail kernel).
+std::vector<std::vector<size_t>> branches(const sgm_plan *p) {
+  std::vector<std::vector<size_t>> br;
+  std::vector<size_t> rest;
+  for (int nm = 6; nm >= 3; --nm)
+    for (size_t i = 0; i < p->bins.size(); ++i)
+      if (p->bins[i].nm == nm) br.push_back({i});
+  for (size_t i = 0; i < p->bins.size(); ++i)
+    if (p->bins[i].nm < 3 || p->bins[i].nm > 6) rest.push_back(i);
+  if (br.empty()) br.push_back({});
+  for (size_t i : rest) br[0].push_back(i);
+  return br;
 }
 
 // Records the counter reset and every bin on stream s. With fork streams
-// (graph capture) bin i runs on aux[i] between a fork and a join event.
+// (graph capture) each branch runs on its own stream between a fork and a
+// join event.
 hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hipStream_t s,
                        bool fork) {
-  hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
+  hipError_t e = reset_counters(p, s);
   if (e != hipSuccess) return e;
-  const size_t nb = p->bins.size();
-  if (!fork || nb < 2 || p->aux.size() < nb || p->ev.size() < nb + 1) {
-    for (size_t i = 0; i < nb; ++i)
-      if ((e = launch_one_bin(p, i, a, P, s)) != hipSuccess) return e;
+  const std::vector<std::vector<size_t>> br = branches(p);
+  const size_t nbr = br.size();
+  if (!fork || nbr < 2 || p->aux.size() < nbr || p->ev.size() < nbr + 1) {
+    for (const auto &b : br)
+      for (size_t i : b)
+        if ((e = launch_one_bin(p, i, a, P, s)) != hipSuccess) return e;
     return hipSuccess;
   }
   if ((e = hipEventRecord(p->ev[0], s)) != hipSuccess) return e;
-  // most materials first: those bins carry the longest iteration caps
-  // (128 x nmat), so their slowest cells start early and overlap the rest
-  for (size_t ii = nb; ii-- > 0;) {
-    const size_t i = ii;
-    if ((e = hipStreamWaitEvent(p->aux[i], p->ev[0], 0)) != hipSuccess) return e;
-    if ((e = launch_one_bin(p, i, a, P, p->aux[i])) != hipSuccess) return e;
-    if ((e = hipEventRecord(p->ev[i + 1], p->aux[i])) != hipSuccess) return e;
+  for (size_t k = 0; k < nbr; ++k) {
+    if ((e = hipStreamWaitEvent(p->aux[k], p->ev[0], 0)) != hipSuccess) return e;
+    for (size_t i : br[k])
+      if ((e = launch_one_bin(p, i, a, P, p->aux[k])) != hipSuccess) return e;
+    if ((e = hipEventRecord(p->ev[k + 1], p->aux[k])) != hipSuccess) return e;
   }
-  for (size_t i = 0; i < nb; ++i)
-    if ((e = hipStreamWaitEvent(s, p->ev[i + 1], 0)) != hipSuccess) return e;
+  for (size_t k = 0; k < nbr; ++k)
+    if ((e = hipStreamWaitEvent(s, p->ev[k + 1], 0)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -224,7 +353,8 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
   }
   (void)other;
   default_grids(p->bins);
-  if (hipMalloc(&p->d_counters, sizeof(int) * std::max<size_t>(1, p->bins.size())) !=
+  if (alloc_park(p, keys) != hipSuccess ||
+      hipMalloc(&p->d_counters, sizeof(int) * std::max<size_t>(1, p->bins.size())) !=
           hipSuccess ||
       hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess) {
     sgm_plan_destroy(p);
@@ -310,6 +440,16 @@ int sgm_plan_reorder(sgm_plan *p, const int *cost, sgm_stream_t stream) {
   if (hipMemcpy(p->d_sorted, out.data(), sizeof(int) * p->ncell, hipMemcpyHostToDevice) !=
       hipSuccess)
     return SGM_E_HIP;
+  // the straggler tables follow the positions: rebuild them for the new order
+  std::vector<unsigned long long> key_of(p->ncell), fk(p->ncell);
+  for (long j = 0; j < p->ncell; ++j) key_of[ids[j]] = keys[j];
+  for (long j = 0; j < p->ncell; ++j) fk[j] = key_of[out[j]];
+  free_park(p);
+  for (Bin &b : p->bins) b.park = ParkArgs{};
+  p->d_run_of = p->d_run_base = p->d_run_cap = p->d_run_count = nullptr;
+  p->d_wave_run = p->d_wave_first = nullptr;
+  if (alloc_park(p, fk) != hipSuccess) return SGM_E_HIP;
+  p->have_key = false;
   return SGM_OK;
 }
 
@@ -471,7 +611,29 @@ int sgm_plan_execute_guided(sgm_plan *p, double *rho, double *vfrac, double *sie
                         params, status, guess_vfrac, guess_T, used_guess, (hipStream_t)stream);
 }
 
-// Launches one bin's kernel directly (no graph) — for per-kernel timing.
+int sgm_plan_set_shape(sgm_plan *p, int bin, int block, int cells_per_wave, int grid) {
+  if (!p || bin < 0 || bin >= (int)p->bins.size() || grid < 1 || cells_per_wave < 64 ||
+      (block != 64 && block != 128 && block != 256))
+    return SGM_E_ARG;
+  Bin &b = p->bins[bin];
+  if (b.nm < 2 || b.nm > 6) return SGM_E_ARG;
+  b.block = block;
+  b.cpw = cells_per_wave;
+  b.grid = grid;
+  p->have_key = false;
+  return SGM_OK;
+}
+
+int sgm_plan_set_straggler_threshold(sgm_plan *p, int iterations) {
+  if (!p || iterations < 0) return SGM_E_ARG;
+  if (iterations != p->park_after) {
+    p->park_after = iterations;
+    p->have_key = false;  // the captured graph holds the old setting
+  }
+  return SGM_OK;
+}
+
+// Launches one bin's kernels directly (no graph) — for per-kernel timing.
 int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, double *sie,
                          double *temp, double *press, const double *vfrac_tot,
                          const double *sie_tot, const double *Tnorm, double *lambda,
@@ -484,7 +646,7 @@ int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, doubl
   const MixParamsDev P = params_dev(params);
   PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
                         status);
-  hipError_t e = launch_zero_ints(p->d_counters + bin, 1, s);
+  hipError_t e = reset_counters(p, s);
   if (e != hipSuccess) return SGM_E_HIP;
   e = launch_one_bin(p, (size_t)bin, a, P, s);
   return e == hipSuccess ? SGM_OK : SGM_E_HIP;
@@ -499,6 +661,7 @@ int sgm_plan_destroy(sgm_plan *p) {
   if (p->d_mat) hipFree(p->d_mat);
   if (p->d_sorted) hipFree(p->d_sorted);
   if (p->d_counters) hipFree(p->d_counters);
+  free_park(p);
   if (p->cap_stream) hipStreamDestroy(p->cap_stream);
   for (hipStream_t q : p->aux) hipStreamDestroy(q);
   for (hipEvent_t ev : p->ev) hipEventDestroy(ev);

@@ -11,6 +11,8 @@
 
 #include "sgm_eos.hpp"
 
+#include <type_traits>
+
 namespace sgm {
 
 // MixParams (mixed_cell_models.hpp:43-78) as a flat POD for kernel arguments
@@ -415,6 +417,36 @@ struct PteBase {
 #pragma unroll
     for (int m = 0; m < NM; ++m) u[m] = sie[m] * robust::ratio(rhobar[m], uscale);
   }
+  // Every field of the solver state that lives from one Newton iteration to
+  // the next (the Jacobian, the step and the line-search copies are rewritten
+  // before they are read; the mixing parameters P come from the kernel
+  // arguments), in a fixed order: a cell can be saved between iterations and
+  // resumed bit for bit.
+  template <class F>
+  SGM_HD constexpr void VisitBase(F &f) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      f(eidx[m]);
+      f(lam[m].present);
+      f(lam[m].lRho);
+      f(lam[m].lT);
+      f(rho[m]);
+      f(vfrac[m]);
+      f(sie[m]);
+      f(press[m]);
+      f(u[m]);
+      f(rhobar[m]);
+    }
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) f(res[i]);
+    f(vfrac_total);
+    f(sie_total);
+    f(rho_total);
+    f(uscale);
+    f(utotal_scale);
+    f(Tnorm);
+    f(err);
+  }
   SGM_HD Real ResidualNorm() const {
     NeumaierSum s;
 #pragma unroll
@@ -637,6 +669,12 @@ struct PteRhoT : PteBase<NM, NM> {
     return B::ResidualNorm();
   }
   SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  template <class F>
+  SGM_HD constexpr void Visit(F &f) {
+    B::VisitBase(f);
+    f(Tequil);
+    f(ms);
+  }
   SGM_HD Real Pout(int m) const { return press[m] * uscale; }
 };
 
@@ -771,6 +809,12 @@ struct PtePT : PteBase<NM, 2> {
     return B::ResidualNorm();
   }
   SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  template <class F>
+  SGM_HD constexpr void Visit(F &f) {
+    B::VisitBase(f);
+    f(Pequil);
+    f(Tequil);
+  }
   SGM_HD Real Pout(int) const { return Pequil * uscale; }
 };
 
@@ -917,6 +961,11 @@ struct PteFixedT : PteBase<NM, NM> {
     return B::ResidualNorm();
   }
   SGM_HD Real Tout() const { return Tequil; }
+  template <class F>
+  SGM_HD constexpr void Visit(F &f) {
+    B::VisitBase(f);
+    f(Tequil);
+  }
   SGM_HD Real Pout(int m) const { return press[m] * uscale; }
 };
 
@@ -1079,6 +1128,12 @@ struct PteFixedP : PteBase<NM, NM + 1> {
     return B::ResidualNorm();
   }
   SGM_HD Real Tout() const { return Tequil * Tnorm; }
+  template <class F>
+  SGM_HD constexpr void Visit(F &f) {
+    B::VisitBase(f);
+    f(Tequil);
+    f(Pequil);
+  }
   SGM_HD Real Pout(int m) const { return press[m] * uscale; }
 };
 
@@ -1094,6 +1149,51 @@ struct DriverState {
   int niter, small_step_iters, max_iter;
   bool converged, close_enough;
 };
+
+template <class F>
+SGM_HD constexpr void VisitDriver(DriverState &d, CellStatus &st, F &f) {
+  f(d.err);
+  f(d.niter);
+  f(d.small_step_iters);
+  f(d.max_iter);
+  f(d.converged);
+  f(d.close_enough);
+  f(st.converged);
+  f(st.niter);
+  f(st.err);
+  f(st.residual);
+}
+
+// A cell's whole solver and driver state as FP64 words (integers and flags
+// are exact in a double): the batched kernels park cells that are still
+// iterating after a set number of Newton iterations and a tail kernel
+// resumes them. ParkWords bounds the word count of every formulation.
+constexpr int ParkWords(int nm, int neq) { return 10 * nm + neq + 24; }
+template <class Sys>
+SGM_HD void ParkSave(Sys &s, DriverState &d, CellStatus &st, double *w) {
+  int k = 0;
+  auto put = [&](auto &x) { w[k++] = (double)x; };
+  s.Visit(put);
+  VisitDriver(d, st, put);
+}
+template <class Sys>
+SGM_HD void ParkLoad(Sys &s, DriverState &d, CellStatus &st, const double *w) {
+  int k = 0;
+  auto get = [&](auto &x) { x = (typename std::remove_reference<decltype(x)>::type)w[k++]; };
+  s.Visit(get);
+  VisitDriver(d, st, get);
+}
+template <class Sys>
+constexpr int ParkWordCount() {
+  Sys s{};
+  DriverState d{};
+  CellStatus st{};
+  int k = 0;
+  auto count = [&](auto &) { ++k; };
+  s.Visit(count);
+  VisitDriver(d, st, count);
+  return k;
+}
 
 template <int NM, class S>
 SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &st) {
@@ -1130,20 +1230,145 @@ SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
     return true;
   }
   const Real gradfdx = -2.0 * scale * d.err;
-  scale = 1.0;
   const Real err_old = d.err;
-  Real err = s.TestUpdate(scale, true);
-  if (err > err_old + p.line_search_alpha * gradfdx) {
-    scale = 0.5;
-    const Real err_mid = s.TestUpdate(scale, false);
-    if (err_mid < err && err_mid < err_old)
-      scale = 0.75 + 0.5 * robust::ratio(err_mid - err, err - 2.0 * err_mid + err_old);
-    for (int li = 0; li < p.line_search_max_iter; li++) {
-      err = s.TestUpdate(scale, false);
-      if (err < err_old + p.line_search_alpha * scale * gradfdx) break;
+  // The reference's line search -- the full step; if it does not decrease
+  // the residual enough, the half step, a quadratic-fit scale and up to
+  // line_search_max_iter backtracking steps -- written as one loop around a
+  // single TestUpdate call, so the (large, per-material) update code exists
+  // once in the kernel instead of three times. The sequence of trial scales,
+  // the accepted state and err are those of the reference's nested form.
+  scale = 1.0;
+  Real err = 0, err_full = 0;
+  int stage = 0, li = 0;
+  for (;;) {
+    const Real e = s.TestUpdate(scale, stage == 0);
+    if (stage == 0) {  // the full step
+      err = err_full = e;
+      if (!(e > err_old + p.line_search_alpha * gradfdx)) break;
+      stage = 1;
+      scale = 0.5;
+    } else if (stage == 1) {  // the half step
+      scale = 0.5;
+      if (e < err_full && e < err_old)
+        scale = 0.75 + 0.5 * robust::ratio(e - err_full, err_full - 2.0 * e + err_old);
+      if (p.line_search_max_iter <= 0) break;  // the half step's state, err of the full step
+      stage = 2;
+    } else {  // backtracking
+      err = e;
+      if (e < err_old + p.line_search_alpha * scale * gradfdx) break;
       scale *= p.line_search_fac;
+      if (++li >= p.line_search_max_iter) break;
     }
   }
+  d.err = err;
+  s.Fixup();
+  if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&
+      d.close_enough) {
+    d.converged = true;
+    return true;
+  }
+  ++d.niter;
+  return d.niter >= d.max_iter;
+}
+
+// Every lane of a group takes lane `src`'s copy of the cell state.
+struct GroupTake {
+  int src;
+  template <class T>
+  __device__ __forceinline__ void operator()(T &x) const {
+    x = (T)__shfl((double)x, src, 64);
+  }
+};
+template <class S>
+__device__ __forceinline__ void group_take(S &s, DriverState &d, CellStatus &st, int src) {
+  GroupTake take{src};
+  s.Visit(take);
+  VisitDriver(d, st, take);
+}
+
+// Whether one TestUpdate's result depends only on the state before the line
+// search and its own scale (so trial steps can run on separate lanes):
+// RhoT, FixedT and FixedP evaluate each material at the trial state; PT's
+// P-T inversions start from the previous trial's densities.
+template <class S>
+struct PureTrialSteps : std::true_type {};
+template <int NM>
+struct PureTrialSteps<PtePT<NM>> : std::false_type {};
+
+// One pass of the PTESolver loop body (pte_iterate) by a group of G lanes
+// (lanes base .. base+G-1, this lane g) that hold identical copies of one
+// cell. Everything up to the line search runs on every lane; the line
+// search's trial updates run side by side: lane 1 the half step and every
+// other lane the full step, then lane k the k-th scale of the backtracking
+// loop. The accepted trial is the one the sequential loop would stop at, and
+// every lane takes its state, so the result is bit-identical to pte_iterate.
+// Needs G >= max(2, line_search_max_iter); every branch below depends only
+// on values shared by the whole group, so its lanes stay converged.
+template <int NM, int G, class S>
+__device__ __forceinline__ bool pte_iterate_group(S &s, DriverState &d, CellStatus &st, int g,
+                                                  int base) {
+  const MixParamsDev &p = s.P;
+  if (d.niter >= d.max_iter) return true;
+  st.niter = d.niter;
+  s.CheckPTE(d.converged, d.close_enough);
+  if (d.converged) return true;
+  s.Jacobian();
+  if (!s.Solve()) {
+    d.converged = false;
+    s.err |= ERRB_LINSOLVE;
+    return true;
+  }
+  Real scale = smin(1.0, s.ScaleDx());
+  if (fabs(scale) < p.pte_small_step_thresh) d.small_step_iters++;
+  if (d.small_step_iters >= p.pte_small_step_tries) {
+    d.converged = false;
+    s.err |= ERRB_SMALLSTEP;
+    return true;
+  }
+  const Real gradfdx = -2.0 * scale * d.err;
+  const Real err_old = d.err;
+  // trial 1: the full step (lane 1: the half step)
+  const Real e1 = s.TestUpdate(g == 1 ? 0.5 : 1.0, true);
+  const Real err_full = __shfl(e1, base, 64);
+  const Real err_mid = __shfl(e1, base + 1, 64);
+  const bool search = err_full > err_old + p.line_search_alpha * gradfdx;
+  Real sc0 = 0.5;
+  if (err_mid < err_full && err_mid < err_old)
+    sc0 = 0.75 + 0.5 * robust::ratio(err_mid - err_full, err_full - 2.0 * err_mid + err_old);
+  // trial 2: lane k the k-th scale of the backtracking loop
+  Real ek = 0;
+  if (search && g < p.line_search_max_iter) {
+    Real sk = sc0;
+    for (int k = 0; k < g; ++k) sk *= p.line_search_fac;
+    ek = s.TestUpdate(sk, false);
+  }
+  Real ks[G];
+#pragma unroll
+  for (int k = 0; k < G; ++k) ks[k] = __shfl(ek, base + k, 64);
+  int src = 0;
+  Real err = err_full;
+  if (search) {
+    if (p.line_search_max_iter <= 0) {
+      src = 1;  // the state of the half step stays, with the full step's error
+    } else {
+      int chosen = p.line_search_max_iter - 1;
+      Real sc = sc0;
+      bool found = false;
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        if (!found && k < p.line_search_max_iter) {
+          if (ks[k] < err_old + p.line_search_alpha * sc * gradfdx) {
+            chosen = k;
+            found = true;
+          }
+          sc *= p.line_search_fac;
+        }
+      }
+      src = chosen;
+      err = pick<G>(ks, chosen);
+    }
+  }
+  group_take(s, d, st, base + src);
   d.err = err;
   s.Fixup();
   if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&

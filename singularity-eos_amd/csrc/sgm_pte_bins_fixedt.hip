@@ -3,7 +3,11 @@
 namespace sgm {
 hipError_t launch_pte_bin_fixedt(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells,
                              long n, const long *range, int *counter, int block, int cpw,
-                             int grid, hipStream_t s) {
-  return kern::bin_launch<2>(nm, a, P, cells, n, counter, block, cpw, grid, s, range);
+                             int grid, const ParkArgs *park, hipStream_t s) {
+  return kern::bin_launch<2>(nm, a, P, cells, n, counter, block, cpw, grid, s, range, park);
+}
+hipError_t launch_pte_tail_fixedt(int nm, const PteArgs &a, const MixParamsDev &P,
+                              const ParkArgs &park, hipStream_t s) {
+  return kern::tail_launch<2>(nm, a, P, park, s);
 }
 } // namespace sgm

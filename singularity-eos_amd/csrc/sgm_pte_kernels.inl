@@ -205,14 +205,21 @@ constexpr int REFILL_BATCH = 1;
 // slowest cell. Cells come from a per-wave reserve that the wave
 // refills `chunk` cells at a time with one atomic on the bin counter; the
 // sorted bin order keeps a wave's cells on the same material signature.
-// One wave per SIMD is allowed: the whole NM-material state then fits in the
-// 512 registers (VGPRs + AGPRs) of a lane instead of spilling to scratch.
+// Two waves per SIMD (256 registers per lane): a wave's dependent FP64
+// chains then hide behind the other wave's. For 3..6 materials a few of the
+// cell's values spill to scratch (the 2-material state fits); measured on the
+// uneven 2^20-cell workload this is faster and steadier than one wave per
+// SIMD with the whole state in 512 registers.
+#ifndef SGM_BIN_WAVES_PER_EU
+#define SGM_BIN_WAVES_PER_EU 2
+#endif
 template <int NM, int METHOD>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDev P,
+__attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
+                                   SGM_BIN_WAVES_PER_EU))) pte_bin_kernel(PteArgs a, MixParamsDev P,
                                                       const int *__restrict__ cells,
                                                       long ncells, int *counter, int chunk,
-                                                      const long *range) {
+                                                      const long *range, ParkArgs pk) {
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
   if (range) {  // bin bounds computed on the device: {start, count} into cells
@@ -225,8 +232,9 @@ __attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDe
   SysOf<NM, METHOD> s;
   DriverState d;
   CellStatus st;
-  long c = -1;
+  long c = -1, jpos = -1;
   int off = 0;
+  bool park_ok = pk.after > 0;  // this lane may still park its cell
   bool active = false;  // this lane holds a cell that is still iterating
   bool done = false;    // the bin is exhausted for this wave
   long rbase = 0, rend = 0;  // wave-uniform reserve [rbase, rend) of bin positions
@@ -258,6 +266,7 @@ __attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDe
       if (!active && !done) {
         if (j >= 0 && j < ncells) {
           c = cells[j];
+          jpos = j;
           const int nm = (c >= 0 && c < a.ncell) ? cell_slice(a, c, off) : -1;
           if (nm == NM) {
             Real Tin[NM];
@@ -281,8 +290,68 @@ __attribute__((amdgpu_waves_per_eu(1, 1))) pte_bin_kernel(PteArgs a, MixParamsDe
       pte_end<NM>(s, d, st);
       store_cell<NM>(s, st, a, c, off);
       active = false;
+    } else if (active && park_ok && d.niter >= pk.after) {
+      // a straggler: save it in its signature run's records for the tail
+      // kernel and free the lane
+      const int r = pk.run_of[jpos];
+      const int i = atomicAdd(pk.run_count + r, 1);
+      if (i < pk.run_cap[r]) {
+        const long slot = (long)pk.run_base[r] + i;
+        ParkSave(s, d, st, pk.rec + slot * pk.stride);
+        pk.cell_of[slot] = (int)c;
+        active = false;
+      }
+      park_ok = false;
     }
   }
+}
+
+// Resumes the parked cells of one bin: one 64-lane block per tail wave, each
+// wave on records of a single signature run. With TailGroup lanes per cell
+// (G) a wave holds 64 / G cells and each group runs its cell's line-search
+// trials in parallel; otherwise one cell per lane. Every cell runs to the end
+// of its Newton loop.
+template <int NM, int METHOD>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
+                                   SGM_BIN_WAVES_PER_EU))) pte_tail_kernel(PteArgs a,
+                                                                      MixParamsDev P,
+                                                                      ParkArgs pk) {
+  constexpr int G = TailGroup(METHOD, NM);
+  const int w = blockIdx.x;
+  if (w >= pk.nwaves) return;
+  const int r = pk.wave_run[w];
+  const int nr = min(pk.run_count[r], pk.run_cap[r]);
+  const int first = pk.wave_first[w];
+  if (first >= nr) return;  // the whole wave is empty
+  extern __shared__ EosDev lds[];
+  stage_registry(lds, a.d_eos, a.nreg);
+  const int lane = (int)threadIdx.x;
+  const int g = lane % G, base = lane - g;
+  const int i = first + lane / G;
+  if (i >= nr) return;  // whole groups leave together
+  const long slot = (long)pk.run_base[r] + i;
+  const long c = pk.cell_of[slot];
+  const int off = a.offsets[c];
+  SysOf<NM, METHOD> s;
+  DriverState d;
+  CellStatus st;
+  ParkLoad(s, d, st, pk.rec + slot * pk.stride);
+  s.P = P;
+  bool grouped = false;
+  if constexpr (G > 1) {
+    grouped = P.line_search_max_iter <= G;
+    if (grouped)
+      while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
+      }
+  }
+  if (!grouped) {
+    if (g != 0) return;
+    while (!pte_iterate<NM>(s, d, st)) {
+    }
+  }
+  pte_end<NM>(s, d, st);
+  if (g == 0) store_cell<NM>(s, st, a, c, off);
 }
 
 template <int METHOD>
@@ -299,12 +368,16 @@ hipError_t plain_launch(const PteArgs &a, const MixParamsDev &P, hipStream_t s) 
 template <int METHOD>
 hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
                       int *counter, int block, int cpw, int grid, hipStream_t s,
-                      const long *range = nullptr) {
+                      const long *range = nullptr, const ParkArgs *park = nullptr) {
   if (n <= 0 && !range) return hipSuccess;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+  const ParkArgs pk = park ? *park : ParkArgs{};
 #define SGM_BIN(K)                                                                             \
   case K:                                                                                      \
-    pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range);   \
+    static_assert(ParkWordCount<SysOf<K, METHOD>>() <= ParkWords(K, K + 1), "park record");    \
+    if (pk.after > 0 && pk.stride < ParkWords(K, K + 1)) return hipErrorInvalidValue;          \
+    pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range,    \
+                                                       pk);                                    \
     break;
   switch (nm) {
     SGM_BIN(2)
@@ -315,6 +388,24 @@ hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int
   default: return hipErrorInvalidValue;
   }
 #undef SGM_BIN
+  return hipGetLastError();
+}
+
+template <int METHOD>
+hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const ParkArgs &pk,
+                       hipStream_t s) {
+  if (pk.nwaves <= 0) return hipSuccess;
+  if (pk.stride < ParkWords(nm, nm + 1)) return hipErrorInvalidValue;
+  const size_t lds = (size_t)a.nreg * sizeof(EosDev);
+  const int g = pk.nwaves;
+  switch (nm) {
+  case 2: pte_tail_kernel<2, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
+  case 3: pte_tail_kernel<3, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
+  case 4: pte_tail_kernel<4, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
+  case 5: pte_tail_kernel<5, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
+  case 6: pte_tail_kernel<6, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
+  default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

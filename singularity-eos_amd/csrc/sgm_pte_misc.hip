@@ -1,5 +1,5 @@
-// Small helper kernels for the batched PTE plan: counter reset (a fill
-// kernel, so the reset can sit inside a captured hipGraph) and per-cell sort
+// Small helper kernels for the batched PTE plan: counter resets (fill
+// kernels, so the resets can sit inside a captured hipGraph), per-cell sort
 // keys (material count, then the material signature in cell order).
 #include "sgm_internal.hpp"
 

@@ -282,3 +282,36 @@ def test_reorder_by_cost_keeps_results(sgm, workloads, pool, torch_gpu):
     for k in FIELDS + ("niter",):
         assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
     plan.close()
+
+
+@pytest.mark.parametrize("method", [0, 1, 3])
+def test_straggler_parking_keeps_results(sgm, workloads, pool, torch_gpu, method):
+    """Parking cells between Newton iterations and resuming them in the tail
+    kernel changes only the schedule: with every cell parked after 2
+    iterations, the default threshold, and parking off, each output, flag and
+    iteration count is bit-identical to the plain per-call path."""
+    torch = torch_gpu
+    ctx, _, _ = pool
+    a = workloads.uneven(ctx, 12000, seed=0x5EED0000 + 24)
+    if method == 3:  # fixed pressure: the volume-weighted input pressure per cell
+        cell = np.repeat(np.arange(a["ncell"]), np.diff(a["offsets"]))
+        a["sie_tot"] = np.abs(np.bincount(cell, weights=a["vfrac"] * a["press"])) + 1e5
+    params = sgm.mix_params(tight=method != 3)
+    g = run_plain(torch, ctx, a, method, params)
+    d = dev(torch, a)
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], method)
+    for after in (2, 40, 0, 1):
+        plan.set_straggler_threshold(after)
+        w = {k: d[k].clone() for k in FIELDS}
+        for k in ("conv", "niter", "resid", "err"):
+            d[k].fill_(-7)
+        assert plan.execute(w["rho"], w["vfrac"], w["sie"], w["temp"], w["press"],
+                            d["vfrac_tot"], d["sie_tot"], params=params, conv=d["conv"],
+                            niter=d["niter"], resid=d["resid"], err=d["err"]) == 0
+        torch.cuda.synchronize()
+        out = {k: w[k].cpu().numpy() for k in FIELDS}
+        out.update({k: d[k].cpu().numpy() for k in ("conv", "niter", "resid", "err")})
+        assert_identical(out, g, f"parking after {after}")
+        assert np.array_equal(out["err"], g["err"])
+        assert np.array_equal(out["resid"], g["resid"])
+    plan.close()
