@@ -213,6 +213,38 @@ int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                    double *press, const double *vfrac_tot, const double *P, double *lambda,
                    const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream);
 
+/* ---- closure driver ----------------------------------------------------- */
+/* Replaces get_sg_eos(nmat, ncell, cell_dim, input_int, eos_offsets, eos,
+ * offsets, press, pmax, vol, spvol, sie, temp, bmod, dpde, cv, frac_mass,
+ * frac_vol, frac_ie, frac_bmod, frac_dpde, frac_cv, mass_frac_cutoff)
+ * (eos/singularity_eos.hpp:114-131, eos/get_sg_eos.cpp:35-280): the
+ * host-code caller of the closure. The EOS array of the reference is the
+ * context: eos_offsets (HOST, [nmat]) holds 1-based context material indices.
+ * Every other array is DEVICE memory: offsets [ncell] 1-based cell indices
+ * into the cell_dim-long per-cell arrays; frac_* are column-major
+ * [cell_dim x nmat] (frac(i, m) = frac[i + m * cell_dim]). input_int: -3
+ * rho-T (PTESolverFixedT), -2 rho-P (PTESolverFixedP), -1 P-T (per-material
+ * P-T inversion), 0 / 1 rho-e (PTESolverRhoT). Temperatures are in eV (K =
+ * eV x 1.160451930280894026e4, get_sg_eos.hpp:68). Mass fractions are
+ * normalised per cell, materials at or below mass_frac_cutoff take no part,
+ * and the outputs are the reference's cell averages and per-material
+ * fractions; frac_bmod / frac_dpde may be NULL and, as the reference decides
+ * (get_sg_eos.cpp:61-62), frac_cv is written exactly when frac_dpde is given
+ * (frac_dpde without frac_cv is rejected). Extensions: params (NULL = the
+ * reference's defaults) and pte_converged (NULL or [ncell] per loop index).
+ * Differences: each cell's lambda cache has storage of its own (the
+ * reference places it inside the PTE solver scratch, where with three or
+ * more materials it overlaps the solver's last arrays; only tabulated EOS
+ * write the cache, so results without tables are the same); cells with more
+ * than 8 participating materials are not solved (pte_converged 0). */
+int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
+                   const int *eos_offsets, const int *offsets, double *press, double *pmax,
+                   double *vol, double *spvol, double *sie, double *temp, double *bmod,
+                   double *dpde, double *cv, double *frac_mass, double *frac_vol,
+                   double *frac_ie, double *frac_bmod, double *frac_dpde, double *frac_cv,
+                   double mass_frac_cutoff, const sgm_mix_params *params,
+                   int *pte_converged, sgm_stream_t stream);
+
 /* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
 /* No reference counterpart: a plan bins the cells of a fixed CSR topology by
  * material count (2..6 run register-resident templated kernels) and by

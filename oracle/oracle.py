@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "liboracle.so")
 SRCS = ["oracle_capi.cpp"]
-HDRS = ["sgo_math.hpp", "sgo_eos.hpp", "sgo_pte.hpp"]
+HDRS = ["sgo_math.hpp", "sgo_eos.hpp", "sgo_pte.hpp", "sgo_driver.hpp"]
 CXXFLAGS = ["-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
             "-shared"]
 
@@ -97,6 +97,7 @@ _SIGS = {
     "sgo_vec_fill": (None, [vp, vp, vp, vp, vp, vp, vp, cl, C.c_ulong, vp, ci, vp, ci]),
     "sgo_vec_rhoe_PT": (None, [vp, vp, vp, vp, vp, cl, vp, ci, vp, ci]),
     "sgo_build_gruneisen_table": (None, [vp, cd, cd, ci, cd, cd, ci] + [vp] * 10),
+    "sgo_get_sg_eos": (cl, [ci, ci, ci, ci, vp, vp, ci, vp] + [vp] * 15 + [cd, vp, vp, vp, ci]),
     "sgo_pte_solve": (cl, [ci, cl, vp, vp, vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                            vp, vp, vp, ci]),
     "sgo_max_threads": (ci, []),
@@ -261,6 +262,32 @@ def pte_solve(method, eos_list, offsets, mat_eos, rho, vfrac, sie, temp, press, 
                                 p(sie_tot), p(Tnorm), p(pa), p(lam), p(conv), p(nit), p(res),
                                 p(err), nthreads)
     return nfail, conv, nit, res, err
+
+
+SG_FIELDS = ("press", "pmax", "vol", "spvol", "sie", "temp", "bmod", "dpde", "cv")
+SG_FRAC = ("frac_mass", "frac_vol", "frac_ie", "frac_bmod", "frac_dpde", "frac_cv")
+
+
+def get_sg_eos(eos_list, input_mode, eos_offsets, offsets, arrays, cutoff, params_arr,
+               nthreads=0):
+    """The get_sg_eos driver on the CPU. arrays: dict of the per-cell arrays
+    (SG_FIELDS, length cell_dim) and column-major per-material arrays
+    (SG_FRAC, cell_dim * nmat; the three optional ones may be absent), updated
+    in place. Returns (nfail, err[ncell], converged[ncell])."""
+    nmat = len(eos_offsets)
+    ncell = len(offsets)
+    cell_dim = len(arrays["press"])
+    handles = (C.c_void_p * len(eos_list))(*[e.h for e in eos_list])
+    eo = np.ascontiguousarray(eos_offsets, dtype=np.int32)
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    err = np.zeros(ncell, dtype=np.int32)
+    conv = np.zeros(ncell, dtype=np.int32)
+    pa = np.asarray(params_arr, dtype=np.float64)
+    ptrs = [p(arrays[k]) for k in SG_FIELDS] + [p(arrays.get(k)) for k in SG_FRAC]
+    nfail = lib().sgo_get_sg_eos(nmat, ncell, cell_dim, input_mode, p(eo), handles,
+                                 len(eos_list), p(off), *ptrs, cutoff, p(pa), p(err), p(conv),
+                                 nthreads)
+    return nfail, err, conv
 
 
 def default_params():

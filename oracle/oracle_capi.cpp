@@ -3,6 +3,7 @@
 // bench.py's cpu_baseline leg (as the host baseline, OpenMP over cells in the
 // shape of singularity-eos/eos/get_sg_eos_rho_e.cpp:30-88: one solver object per
 // cell, per-thread scratch, schedule(dynamic,256)).
+#include "sgo_driver.hpp"
 #include "sgo_pte.hpp"
 
 #include <omp.h>
@@ -338,6 +339,61 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
       if (out_residual) out_residual[c] = st.residual;
       if (out_err) out_err[c] = st.err;
       nfail += st.converged ? 0 : 1;
+    }
+  }
+  return nfail;
+}
+
+// ---- get_sg_eos driver --------------------------------------------------
+// eos_handles[neos] is the material pool; eos_offsets[nmat] are 1-based pool
+// indices; offsets[ncell] are 1-based cell indices; per-material arrays are
+// column-major [cell_dim x nmat]. out_err / out_conv: per loop index, may be
+// null. Returns the number of unconverged PTE solves.
+long sgo_get_sg_eos(int nmat, int ncell, int cell_dim, int input, const int *eos_offsets,
+                    void **eos_handles, int neos, const int *offsets, double *press,
+                    double *pmax, double *vol, double *spvol, double *sie, double *temp,
+                    double *bmod, double *dpde, double *cv, double *frac_mass,
+                    double *frac_vol, double *frac_ie, double *frac_bmod, double *frac_dpde,
+                    double *frac_cv, double mass_frac_cutoff, const double *params_arr,
+                    int *out_err, int *out_conv, int nthreads) {
+  std::vector<EOS> pool(neos);
+  for (int i = 0; i < neos; ++i) pool[i] = *static_cast<EOS *>(eos_handles[i]);
+  SgEosArgs a;
+  a.nmat = nmat;
+  a.ncell = ncell;
+  a.cell_dim = cell_dim;
+  a.input = input;
+  a.eos_offsets = eos_offsets;
+  a.pool = pool.data();
+  a.offsets = offsets;
+  a.press = press;
+  a.pmax = pmax;
+  a.vol = vol;
+  a.spvol = spvol;
+  a.sie = sie;
+  a.temp = temp;
+  a.bmod = bmod;
+  a.dpde = dpde;
+  a.cv = cv;
+  a.frac_mass = frac_mass;
+  a.frac_vol = frac_vol;
+  a.frac_ie = frac_ie;
+  a.frac_bmod = frac_bmod;
+  a.frac_dpde = frac_dpde;
+  a.frac_cv = frac_cv;
+  a.mass_frac_cutoff = mass_frac_cutoff;
+  a.params = params_from_array(params_arr);
+  long nfail = 0;
+#pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : nfail)
+  {
+    SgEosScratch w(nmat);
+#pragma omp for schedule(dynamic, 64)
+    for (int c = 0; c < ncell; ++c) {
+      bool conv = true;
+      const int e = sg_eos_cell(a, c, w, &conv);
+      if (out_err) out_err[c] = e;
+      if (out_conv) out_conv[c] = conv ? 1 : 0;
+      nfail += conv ? 0 : 1;
     }
   }
   return nfail;

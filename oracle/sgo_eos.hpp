@@ -856,6 +856,22 @@ struct SpinerRhoT {
     setlog(lambda, lRho, lT);
     return CvFromlRholT_(lRho, lT, locRhoT(lRho, lT));
   }
+  // GruneisenParamFromDensityTemperature — eos_spiner_rho_temp.hpp:877-895
+  Real gruneisen_rT(Real rho, Real T, Real *lambda) const {
+    const Real lRho = lRho_(rho), lT = lT_(T);
+    setlog(lambda, lRho, lT);
+    const TableStatus w = locRhoT(lRho, lT);
+    if (w == OffBottom) return robust::ratio(std::abs(dPdECold_.interp(lRho)), std::abs(rho));
+    if (w == OffTop) return gm1Max_.interp(lRho);
+    return robust::ratio(std::abs(dPdE_.interp(lRho, lT)), std::abs(rho));
+  }
+  // the temperature of ValuesAtReferenceState (eos_spiner_rho_temp.hpp:671-677)
+  Real TNormal() const {
+    constexpr Real ROOM_TEMPERATURE = 293;  // base/constants.hpp:41
+    const Real lTN = lT_(ROOM_TEMPERATURE);
+    if (!(lTMin_ < lTN && lTN < lTMax_)) return T_(0.5 * (lTMin_ + lTMax_));
+    return ROOM_TEMPERATURE;
+  }
   Real cv_re(Real rho, Real sie, Real *lambda) const {
     TableStatus w;
     const Real lRho = lRho_(rho);
@@ -1036,6 +1052,31 @@ struct EOS {
     case K_GRUN: return gr.bmod_re(rho, sie);
     case K_VINET: return vn.bmod_re(rho, sie);
     default: return tb->bmod_re(rho, sie, lam);
+    }
+  }
+  // GruneisenParamFromDensityTemperature: IdealGas / StiffGas gm1
+  // (eos_ideal.hpp:140-144, eos_stiff.hpp:135-139), Gruneisen Gamma(min(rho,
+  // rho_max)) (eos_gruneisen.hpp:134-138), Vinet A0 B0 / (Cv0 rho)
+  // (eos_vinet.hpp:110-114), the table's dPdE / rho (see SpinerRhoT)
+  Real gruneisen_rT(Real rho, Real T, Real *lam) const {
+    switch (kind) {
+    case K_IDEAL: return ig.gm1;
+    case K_STIFF: return sg.gm1;
+    case K_GRUN: return gr.Gamma(std::min(rho, gr.rho_max));
+    case K_VINET: return robust::ratio(vn.A0 * vn.B0, vn.Cv0 * rho);
+    default: return tb->gruneisen_rT(rho, T, lam);
+    }
+  }
+  // the temperature of ValuesAtReferenceState: room temperature for IdealGas
+  // and StiffGas (eos_ideal.hpp:224, eos_stiff.hpp:42), T0 for Gruneisen and
+  // Vinet (eos_gruneisen.hpp:558, eos_vinet.hpp:408), TNormal for the table
+  Real T_reference() const {
+    switch (kind) {
+    case K_IDEAL:
+    case K_STIFF: return 293;
+    case K_GRUN: return gr.T0;
+    case K_VINET: return vn.T0;
+    default: return tb->TNormal();
     }
   }
   // EosBase::MinInternalEnergyFromDensity default = e(rho, 0) (eos_base.hpp:598-603);

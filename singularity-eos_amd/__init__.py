@@ -114,6 +114,8 @@ _SIGS = {
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_get_sg_eos": (ci, [vp, ci, ci, ci, ci, vp, vp] + [vp] * 15 +
+                       [cd, C.POINTER(MixParams), vp, vp]),
     "sgm_plan_create": (ci, [vp, cl, cl, vp, vp, ci, C.POINTER(vp), vp]),
     "sgm_plan_set_straggler_threshold": (ci, [vp, ci]),
     "sgm_plan_set_shape": (ci, [vp, ci, ci, ci, ci]),
@@ -294,6 +296,25 @@ class Context:
         return f(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids), ptr(rho), ptr(vfrac), ptr(sie),
                  ptr(temp), ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm), ptr(lam), pp,
                  st, stream)
+
+
+SG_FIELDS = ("press", "pmax", "vol", "spvol", "sie", "temp", "bmod", "dpde", "cv")
+SG_FRAC = ("frac_mass", "frac_vol", "frac_ie", "frac_bmod", "frac_dpde", "frac_cv")
+
+
+def get_sg_eos(ctx, input_mode, eos_offsets, offsets, arrays, cutoff, params=None,
+               converged=None, stream=0):
+    """sgm_get_sg_eos: eos_offsets is a host sequence of 1-based context
+    material indices; offsets (1-based cell indices) and every array in
+    `arrays` (SG_FIELDS per cell, SG_FRAC column-major per material, the last
+    three optional) are device tensors, updated in place."""
+    eo = (C.c_int * len(eos_offsets))(*[int(x) for x in eos_offsets])
+    ptrs = [ptr(arrays[k]) for k in SG_FIELDS] + [ptr(arrays.get(k)) for k in SG_FRAC]
+    return lib().sgm_get_sg_eos(ctx.h, len(eos_offsets), int(offsets.numel()),
+                                int(arrays["press"].numel()), int(input_mode), eo, ptr(offsets),
+                                *ptrs, float(cutoff),
+                                C.byref(params) if params is not None else None,
+                                ptr(converged), stream)
 
 
 class Plan:

@@ -13,8 +13,7 @@ bool build_table_host(const sgm_table_desc &in, TableHost &T);
 hipError_t upload_table(TableHost &T);
 } // namespace sgm
 
-namespace {
-
+namespace sgm {
 int sync_registry(sgm_ctx *ctx) {
   if (ctx->host_only) return SGM_OK;
   if (!ctx->dirty) return SGM_OK;
@@ -23,19 +22,6 @@ int sync_registry(sgm_ctx *ctx) {
     return SGM_E_HIP;
   ctx->dirty = false;
   return SGM_OK;
-}
-
-int check_mat(const sgm_ctx *ctx, int m) {
-  if (!ctx) return SGM_E_ARG;
-  if (m < 0 || m >= ctx->nmat || !ctx->is_set[m]) return SGM_E_MAT;
-  return SGM_OK;
-}
-
-// every entry point that launches work starts here
-int check_launch(const sgm_ctx *ctx, int m) {
-  const int r = check_mat(ctx, m);
-  if (r != SGM_OK) return r;
-  return ctx->host_only ? SGM_E_NOGPU : SGM_OK;
 }
 
 MixParamsDev to_dev(const sgm_mix_params *p) {
@@ -48,6 +34,22 @@ MixParamsDev to_dev(const sgm_mix_params *p) {
   static_assert(sizeof(MixParamsDev) == sizeof(sgm_mix_params), "MixParams layout");
   std::memcpy(&d, &q, sizeof(d));
   return d;
+}
+} // namespace sgm
+
+namespace {
+
+int check_mat(const sgm_ctx *ctx, int m) {
+  if (!ctx) return SGM_E_ARG;
+  if (m < 0 || m >= ctx->nmat || !ctx->is_set[m]) return SGM_E_MAT;
+  return SGM_OK;
+}
+
+// every entry point that launches work starts here
+int check_launch(const sgm_ctx *ctx, int m) {
+  const int r = check_mat(ctx, m);
+  if (r != SGM_OK) return r;
+  return ctx->host_only ? SGM_E_NOGPU : SGM_OK;
 }
 
 int set_common(sgm_ctx *ctx, int m, const EosDev &e) {

@@ -1,0 +1,399 @@
+// get_sg_eos: the closure driver a hydro code calls once per step
+// (singularity-eos/eos/get_sg_eos.cpp:35-280 with get_sg_eos_functors.hpp and
+// the per-mode bodies get_sg_eos_{rho_e,rho_t,rho_p,p_t}.cpp). On the device
+// it is four passes over the cells around the closure kernels:
+//
+//  1. prologue: normalise each cell's mass fractions in place, apply the
+//     mass-fraction cutoff, count the participating materials (the reference's
+//     init_functor);
+//  2. an exclusive scan of the counts gives a CSR layout of the participating
+//     (cell, material) entries;
+//  3. fill: the PTE inputs of every entry (densities at the cell's specific
+//     volume, the mode's temperature / pressure / energy guesses, zeroed
+//     lambda caches) and the list of cells with two or more materials;
+//  4. the plain PTE list kernel (RhoT, FixedT or FixedP by mode) over that
+//     list, then the epilogue: the single-material paths, the P-T mode's
+//     per-material inversions, and the reference's final_functor (cell
+//     averages, per-material bulk modulus, heat capacity, Gruneisen
+//     parameter, volume and energy fractions, pmax).
+//
+// Per-material arrays are column-major [cell_dim x nmat] as the reference's
+// Kokkos LayoutLeft views: x(i, m) = x[i + m * cell_dim]. The lambda cache of
+// each entry has storage of its own (the reference places it inside the
+// solver scratch, see sgm.h).
+#include "sgm_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cstring>
+#include <vector>
+
+using namespace sgm;
+
+namespace {
+
+constexpr Real EV2K = 1.160451930280894026e4;  // get_sg_eos.hpp:68
+constexpr int MAX_DRIVER_MATS = 64;
+
+enum { IN_RHO_T = -3, IN_RHO_P = -2, IN_P_T = -1, IN_NORM_RHO_E = 0, IN_RHO_E = 1 };
+
+struct DrvArgs {
+  int nmat = 0, ncell = 0, input = 1;
+  long cd = 0;
+  int eos_idx[MAX_DRIVER_MATS] = {};  // context material index per material slot
+  const int *offsets = nullptr;       // 1-based cell index per loop index
+  double *press, *pmax, *vol, *spvol, *sie, *temp, *bmod, *dpde, *cv;
+  double *fm, *fv, *fi, *fb, *fd, *fc;
+  double cutoff = 0;
+  const EosDev *d_eos = nullptr;
+  int nreg = 0;
+  // per loop index
+  int *npte = nullptr;
+  double *mass_sum = nullptr;
+  int *eoff = nullptr;  // [ncell + 1] entry offsets
+  double *cell_in = nullptr, *vtot = nullptr;
+  unsigned long long *list_range = nullptr;  // {0, count} of the multi-material list
+  int *list = nullptr;
+  int *conv = nullptr;  // optional: PTE convergence per loop index
+  // per entry
+  int *emat = nullptr;
+  uint8_t *eid = nullptr;
+  double *rho = nullptr, *vfrac = nullptr, *esie = nullptr, *etemp = nullptr,
+         *epress = nullptr, *lam = nullptr;
+};
+
+__device__ __forceinline__ Real mults_t(int in) { return in == IN_RHO_T ? 1.0 : 0.0; }
+__device__ __forceinline__ Real mults_p(int in) { return in == IN_RHO_P ? 1.0 : 0.0; }
+__device__ __forceinline__ Real mults_s(int in) {
+  return (in == IN_RHO_E || in == IN_NORM_RHO_E) ? 1.0 : 0.0;
+}
+
+// init_functor, first half (get_sg_eos_functors.hpp:64-100); P-T mode:
+// get_sg_eos_p_t.cpp:40-58
+__global__ void prologue_kernel(DrvArgs a) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= a.ncell) return;
+  const long i = a.offsets[il] - 1;
+  const long cd = a.cd;
+  if (i < 0 || i >= cd) {  // not a cell of the arrays: no materials, left untouched
+    a.npte[il] = 0;
+    a.mass_sum[il] = 1.0;
+    return;
+  }
+  Real ms = 0.0;
+  int n = 0;
+  if (a.input != IN_P_T) {
+    const Real s_mult = mults_s(a.input);
+    for (int m = 0; m < a.nmat; ++m) {
+      ms += a.fm[i + m * cd];
+      a.fv[i + m * cd] = a.fm[i + m * cd];
+      a.fi[i + m * cd] = a.sie[i] * a.fm[i + m * cd] * s_mult;
+    }
+    for (int m = 0; m < a.nmat; ++m) a.fm[i + m * cd] /= ms;
+    for (int m = 0; m < a.nmat; ++m) n += a.fm[i + m * cd] > a.cutoff ? 1 : 0;
+  } else {
+    for (int m = 0; m < a.nmat; ++m) ms += a.fm[i + m * cd];
+    for (int m = 0; m < a.nmat; ++m) a.fm[i + m * cd] /= ms;
+    n = a.nmat;
+  }
+  a.npte[il] = n;
+  a.mass_sum[il] = ms;
+}
+
+// init_functor, second half (:101-113): the PTE inputs of each entry
+__global__ void fill_kernel(DrvArgs a) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= a.ncell) return;
+  const long i = a.offsets[il] - 1;
+  const long cd = a.cd;
+  const int o = a.eoff[il];
+  const int n = a.npte[il];
+  if (a.conv) a.conv[il] = (i >= 0 && i < cd) ? 1 : 0;
+  if (i < 0 || i >= cd) {
+    a.vtot[il] = 0;
+    a.cell_in[il] = 0;
+    return;
+  }
+  if (a.input == IN_P_T) {
+    for (int m = 0; m < a.nmat; ++m) {
+      const int e = o + m;
+      a.emat[e] = m;
+      a.eid[e] = (uint8_t)a.eos_idx[m];
+      a.etemp[e] = a.temp[i] * EV2K;
+      a.epress[e] = a.press[i];
+      a.rho[e] = 0.0;  // density guess of the P-T inversions
+      a.esie[e] = 0.0;
+      a.lam[3L * e] = a.lam[3L * e + 1] = a.lam[3L * e + 2] = 0.0;
+    }
+    return;
+  }
+  const Real t_mult = mults_t(a.input), p_mult = mults_p(a.input), s_mult = mults_s(a.input);
+  Real vsum = 0.;
+  int mp = 0;
+  for (int m = 0; m < a.nmat; ++m) {
+    const Real f = a.fm[i + m * cd];
+    if (!(f > a.cutoff)) continue;
+    const int e = o + mp;
+    a.emat[e] = m;
+    a.eid[e] = (uint8_t)a.eos_idx[m];
+    a.vfrac[e] = f;
+    vsum += a.vfrac[e];
+    a.rho[e] = f / a.spvol[i] / a.vfrac[e];
+    a.etemp[e] = a.temp[i] * EV2K * t_mult;
+    a.epress[e] = a.press[i] * p_mult;
+    a.esie[e] = a.sie[i] * f * s_mult;
+    a.lam[3L * e] = a.lam[3L * e + 1] = a.lam[3L * e + 2] = 0.0;
+    ++mp;
+  }
+  a.vtot[il] = vsum;
+  // the PTE cell input: the cell energy (rho-e), the first entry's
+  // temperature (rho-T) or pressure (rho-P), as the reference passes them
+  a.cell_in[il] = a.input == IN_RHO_T ? (n > 0 ? a.etemp[o] : 0.0)
+                  : a.input == IN_RHO_P ? (n > 0 ? a.epress[o] : 0.0)
+                                        : a.sie[i];
+  if (n > 1) {
+    const unsigned long long k = atomicAdd(a.list_range + 1, 1ull);
+    a.list[k] = il;
+  }
+}
+
+// the single-material paths, the P-T mode's inversions and final_functor
+// (get_sg_eos_functors.hpp:206-282)
+__global__ void epilogue_kernel(DrvArgs a) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= a.ncell) return;
+  const long i = a.offsets[il] - 1;
+  const long cd = a.cd;
+  if (i < 0 || i >= cd) return;
+  const int o = a.eoff[il];
+  const int n = a.npte[il];
+  const Real ms = a.mass_sum[il];
+  auto lam_of = [&](int e) {
+    Lam l;
+    l.present = true;
+    l.lRho = a.lam[3L * e];
+    l.lT = a.lam[3L * e + 1];
+    return l;
+  };
+  Real ft = 0, fs = 0, fp = 0;
+  if (a.input == IN_RHO_E || a.input == IN_NORM_RHO_E) {
+    if (n == 1) {  // get_sg_eos_rho_e.cpp:73-80
+      const EosDev &e = a.d_eos[a.eid[o]];
+      Lam l = lam_of(o);
+      a.etemp[o] = eos_T_re(e, a.rho[o], a.esie[o], l);
+      a.epress[o] = eos_P_rT(e, a.rho[o], a.etemp[o], l);
+      a.lam[3L * o] = l.lRho;
+      a.lam[3L * o + 1] = l.lT;
+    }
+    ft = 1.0, fs = 0.0, fp = 1.0;
+  } else if (a.input == IN_RHO_T || a.input == IN_RHO_P) {
+    Real sie_true = 0.0;
+    if (n > 1) {  // get_sg_eos_rho_t.cpp:75-79, get_sg_eos_rho_p.cpp:73-77
+      for (int mp = 0; mp < n; ++mp) sie_true += a.esie[o + mp] * a.fm[i + a.emat[o + mp] * cd];
+    } else if (n == 1) {
+      const EosDev &e = a.d_eos[a.eid[o]];
+      Lam l = lam_of(o);
+      if (a.input == IN_RHO_T) {  // get_sg_eos_rho_t.cpp:80-91
+        a.esie[o] = eos_e_rT(e, a.rho[o], a.etemp[o], l);
+        sie_true = a.esie[o];
+        a.epress[o] = eos_P_rT(e, a.rho[o], a.etemp[o], l);
+      } else {  // get_sg_eos_rho_p.cpp:78-101
+        const Real r = a.rho[o];
+        auto p_of_t = [&](Real t) { return eos_P_rT(e, r, t, l); };
+        const Real r_temp = eos_T_reference(e);
+        Real temp_i = 0;
+        root::find_root(p_of_t, a.press[i], r_temp, 1.e-10 * r_temp, 1.e10 * r_temp, 1.e-12,
+                        1.e-12, temp_i);
+        a.esie[o] = eos_e_rT(e, r, temp_i, l);
+        sie_true = a.esie[o];
+        a.etemp[o] = temp_i;
+      }
+      a.lam[3L * o] = l.lRho;
+      a.lam[3L * o + 1] = l.lT;
+    }
+    a.sie[i] = sie_true;
+    if (a.input == IN_RHO_T)
+      ft = 0.0, fs = 0.0, fp = 1.0;
+    else
+      ft = 1.0, fs = 0.0, fp = 0.0;
+  } else {  // P-T: get_sg_eos_p_t.cpp:59-90
+    Real vt = 0.0, st = 0.0;
+    int err = 0;
+    for (int m = 0; m < n; ++m) {
+      const int e = o + m;
+      Lam l = lam_of(e);
+      eos_rhoe_PT(a.d_eos[a.eid[e]], a.epress[e], a.etemp[e], a.rho[e], a.esie[e], l, err);
+      a.lam[3L * e] = l.lRho;
+      a.lam[3L * e + 1] = l.lT;
+      a.vfrac[e] = a.fm[i + m * cd] / a.rho[e] * ms;
+      vt += a.vfrac[e];
+      st += a.esie[e] * a.fm[i + m * cd];
+    }
+    a.sie[i] = st;
+    a.vol[i] = vt;
+    a.spvol[i] = a.vol[i] / ms;
+    for (int m = 0; m < n; ++m) a.vfrac[o + m] /= vt;
+  }
+  // final_functor
+  const bool do_t = ft == 1.0, do_s = fs == 1.0, do_p = fp == 1.0;
+  if (do_t) a.temp[i] = 0.0;
+  if (do_p) a.press[i] = 0.0;
+  if (do_s) a.sie[i] = 0.0;
+  a.bmod[i] = 0.0;
+  a.cv[i] = 0.0;
+  a.dpde[i] = 0.0;
+  for (int mp = 0; mp < n; ++mp) {
+    const int e = o + mp;
+    const int m = a.emat[e];
+    const EosDev &eo = a.d_eos[a.eid[e]];
+    Lam l = lam_of(e);
+    a.press[i] += a.epress[e] * a.vfrac[e] * fp;
+    a.temp[i] += a.etemp[e] * a.vfrac[e] * ft;
+    const Real ie_m = a.esie[e] * a.fm[i + m * cd] * ms;
+    a.sie[i] += ie_m * fs;
+    a.fi[i + m * cd] = ie_m;
+    a.fv[i + m * cd] = a.vfrac[e] * a.vol[i];
+    const Real bmod_m = eos_bmod_rT(eo, a.rho[e], a.etemp[e], l);
+    a.bmod[i] += bmod_m * a.vfrac[e];
+    const Real cv_m = EV2K * eos_cv_rT(eo, a.rho[e], a.etemp[e], l);
+    a.cv[i] += cv_m * a.fm[i + m * cd];
+    const Real dpde_m = eos_gruneisen_rT(eo, a.rho[e], a.etemp[e], l);
+    a.dpde[i] += dpde_m * a.vfrac[e];
+    if (a.fb) a.fb[i + m * cd] = bmod_m;
+    if (a.fd) a.fc[i + m * cd] = cv_m;  // written when frac_dpde is given (get_sg_eos.cpp:61-62)
+    if (a.fd) a.fd[i + m * cd] = dpde_m;
+  }
+  if (do_t) a.temp[i] /= EV2K;
+  if (do_s) a.sie[i] /= ms;
+  for (int m = 0; m < a.nmat; ++m) a.fm[i + m * cd] *= ms;
+  a.pmax[i] = a.press[i] > a.pmax[i] ? a.press[i] : a.pmax[i];
+}
+
+__global__ void list_range_kernel(unsigned long long *r) {
+  r[0] = 0;
+  r[1] = 0;
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+}  // namespace
+
+extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
+                              const int *eos_offsets, const int *offsets, double *press,
+                              double *pmax, double *vol, double *spvol, double *sie,
+                              double *temp, double *bmod, double *dpde, double *cv,
+                              double *frac_mass, double *frac_vol, double *frac_ie,
+                              double *frac_bmod, double *frac_dpde, double *frac_cv,
+                              double mass_frac_cutoff, const sgm_mix_params *params,
+                              int *pte_converged, sgm_stream_t stream) {
+  if (!ctx || nmat < 1 || nmat > MAX_DRIVER_MATS || ncell < 0 || cell_dim < ncell ||
+      input_int < IN_RHO_T || input_int > IN_RHO_E || !eos_offsets || !offsets || !press ||
+      !pmax || !vol || !spvol || !sie || !temp || !bmod || !dpde || !cv || !frac_mass ||
+      !frac_vol || !frac_ie || (frac_dpde && !frac_cv))
+    return SGM_E_ARG;
+  if (ctx->host_only) return SGM_E_NOGPU;
+  if (ncell == 0) return SGM_OK;
+  DrvArgs a;
+  a.nmat = nmat;
+  a.ncell = ncell;
+  a.input = input_int;
+  a.cd = cell_dim;
+  for (int m = 0; m < nmat; ++m) {
+    a.eos_idx[m] = eos_offsets[m] - 1;
+    if (a.eos_idx[m] < 0 || a.eos_idx[m] >= ctx->nmat || !ctx->is_set[a.eos_idx[m]])
+      return SGM_E_MAT;
+  }
+  a.offsets = offsets;
+  a.press = press;
+  a.pmax = pmax;
+  a.vol = vol;
+  a.spvol = spvol;
+  a.sie = sie;
+  a.temp = temp;
+  a.bmod = bmod;
+  a.dpde = dpde;
+  a.cv = cv;
+  a.fm = frac_mass;
+  a.fv = frac_vol;
+  a.fi = frac_ie;
+  a.fb = frac_bmod;
+  a.fd = frac_dpde;
+  a.fc = frac_cv;
+  a.cutoff = mass_frac_cutoff;
+  a.conv = pte_converged;
+  if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
+  a.d_eos = ctx->d_eos;
+  a.nreg = ctx->nmat;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<void *> tmp;
+  auto cleanup = [&](int rc) {
+    hipStreamSynchronize(s);
+    for (void *q : tmp) (void)hipFree(q);
+    return rc;
+  };
+  auto track = [&](hipError_t e, void *q) {
+    if (e == hipSuccess) tmp.push_back(q);
+    return e == hipSuccess;
+  };
+  const int nb = (ncell + 255) / 256;
+  if (!track(dalloc(&a.npte, ncell + 1), a.npte) ||
+      !track(dalloc(&a.mass_sum, ncell), a.mass_sum) ||
+      !track(dalloc(&a.eoff, ncell + 1), a.eoff) || !track(dalloc(&a.cell_in, ncell), a.cell_in) ||
+      !track(dalloc(&a.vtot, ncell), a.vtot) || !track(dalloc(&a.list, ncell), a.list) ||
+      !track(dalloc(&a.list_range, 2), a.list_range))
+    return cleanup(SGM_E_HIP);
+  if (hipMemsetAsync(a.npte + ncell, 0, sizeof(int), s) != hipSuccess) return cleanup(SGM_E_HIP);
+  prologue_kernel<<<nb, 256, 0, s>>>(a);
+  if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
+  size_t scan_bytes = 0;
+  void *scan_tmp = nullptr;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
+          hipSuccess ||
+      !track(hipMalloc(&scan_tmp, scan_bytes + 16), scan_tmp) ||
+      hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
+          hipSuccess)
+    return cleanup(SGM_E_HIP);
+  int nnz = 0;
+  if (hipMemcpyAsync(&nnz, a.eoff + ncell, sizeof(int), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return cleanup(SGM_E_HIP);
+  if (!track(dalloc(&a.emat, nnz), a.emat) || !track(dalloc(&a.eid, nnz), a.eid) ||
+      !track(dalloc(&a.rho, nnz), a.rho) || !track(dalloc(&a.vfrac, nnz), a.vfrac) ||
+      !track(dalloc(&a.esie, nnz), a.esie) || !track(dalloc(&a.etemp, nnz), a.etemp) ||
+      !track(dalloc(&a.epress, nnz), a.epress) || !track(dalloc(&a.lam, 3L * nnz), a.lam))
+    return cleanup(SGM_E_HIP);
+  list_range_kernel<<<1, 1, 0, s>>>(a.list_range);
+  fill_kernel<<<nb, 256, 0, s>>>(a);
+  if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
+  if (input_int != IN_P_T) {
+    PteArgs p;
+    p.d_eos = ctx->d_eos;
+    p.nreg = ctx->nmat;
+    p.ncell = ncell;
+    p.nnz = nnz;
+    p.offsets = a.eoff;
+    p.mat_ids = a.eid;
+    p.rho = a.rho;
+    p.vfrac = a.vfrac;
+    p.sie = a.esie;
+    p.temp = a.etemp;
+    p.press = a.epress;
+    p.vfrac_tot = a.vtot;
+    p.sie_tot = a.cell_in;
+    p.lambda = a.lam;
+    p.conv = a.conv;
+    const int method = input_int == IN_RHO_T   ? PTE_FIXED_T
+                       : input_int == IN_RHO_P ? PTE_FIXED_P
+                                               : PTE_RHOT;
+    if (launch_pte_list(method, p, to_dev(params), a.list, ncell, s,
+                        reinterpret_cast<const long *>(a.list_range)) != hipSuccess)
+      return cleanup(SGM_E_HIP);
+  }
+  epilogue_kernel<<<nb, 256, 0, s>>>(a);
+  if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
+  return cleanup(SGM_OK);
+}

@@ -297,6 +297,22 @@ SGM_HD Real cv_rT(const TableDev &t, Real rho, Real T, Lam &lam) {
   lam.set(lRho, lT);
   return cv_from(t, lRho, lT, loc_rho_T(t, lRho, lT));
 }
+// GruneisenParamFromDensityTemperature (eos_spiner_rho_temp.hpp:877-895)
+SGM_HD Real gruneisen_rT(const TableDev &t, Real rho, Real T, Lam &lam) {
+  const Real lRho = lRho_(t, rho), lT = lT_(t, T);
+  lam.set(lRho, lT);
+  const int w = loc_rho_T(t, lRho, lT);
+  if (w == OffBottom) return robust::ratio(fabs(lerp_rho(t, t.dPdECold, lRho)), fabs(rho));
+  if (w == OffTop) return lerp_rho(t, t.gm1Max, lRho);
+  return robust::ratio(fabs(bilerp(t, t.dPdE, lRho, lT)), fabs(rho));
+}
+// the temperature of ValuesAtReferenceState (eos_spiner_rho_temp.hpp:671-677)
+SGM_HD Real T_normal(const TableDev &t) {
+  const Real room = 293;  // ROOM_TEMPERATURE, base/constants.hpp:41
+  const Real lTN = lT_(t, room);
+  if (!(t.lTMin < lTN && lTN < t.lTMax)) return T_(t, 0.5 * (t.lTMin + t.lTMax));
+  return room;
+}
 SGM_HD Real bmod_rT(const TableDev &t, Real rho, Real T, Lam &lam) {
   const Real lRho = lRho_(t, rho), lT = lT_(t, T);
   lam.set(lRho, lT);
@@ -677,6 +693,31 @@ SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   case KIND_GRUN: return grun::bmod_rT(e.m.p, rho, T);
   case KIND_VINET: return vinet::bmod_rT(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
   default: return table::bmod_rT(e.tab, rho, T, lam);
+  }
+}
+// GruneisenParamFromDensityTemperature: IdealGas / StiffGas gm1
+// (eos_ideal.hpp:140-144, eos_stiff.hpp:135-139), Gruneisen Gamma(min(rho,
+// rho_max)) (eos_gruneisen.hpp:134-138), Vinet A0 B0 / (Cv0 rho)
+// (eos_vinet.hpp:110-114), the table's dPdE / rho
+SGM_HD Real eos_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  switch (e.kind) {
+  case KIND_IDEAL:
+  case KIND_STIFF: return e.m.p[1];
+  case KIND_GRUN: return grun::Gamma(e.m.p, smin(rho, e.m.p[10]));
+  case KIND_VINET: return robust::ratio(e.m.p[4] * e.m.p[2], e.m.p[5] * rho);
+  default: return table::gruneisen_rT(e.tab, rho, T, lam);
+  }
+}
+// the temperature of ValuesAtReferenceState: room temperature for IdealGas
+// and StiffGas (eos_ideal.hpp:224, eos_stiff.hpp:42), T0 for Gruneisen and
+// Vinet (eos_gruneisen.hpp:558, eos_vinet.hpp:408), TNormal for the table
+SGM_HD Real eos_T_reference(const EosDev &e) {
+  switch (e.kind) {
+  case KIND_IDEAL:
+  case KIND_STIFF: return 293;
+  case KIND_GRUN: return e.m.p[7];
+  case KIND_VINET: return e.m.p[1];
+  default: return table::T_normal(e.tab);
   }
 }
 SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {

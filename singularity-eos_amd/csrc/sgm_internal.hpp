@@ -186,6 +186,10 @@ hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, 
                             unsigned long long *keys, int *ids, hipStream_t s,
                             const int *cost = nullptr, int threshold = 0);
 int device_cu_count();
+// copies the host registry to the device when it changed (sgm_capi.cpp)
+int sync_registry(sgm_ctx *ctx);
+// MixParams as the kernels take them (NULL = the reference's defaults)
+MixParamsDev to_dev(const sgm_mix_params *p);
 // Gruneisen::ComputeRhoMax (eos_gruneisen.hpp:231-329), host side
 Real gruneisen_compute_rho_max(Real s1, Real s2, Real s3, Real rho0);
 } // namespace sgm

@@ -311,10 +311,16 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
 // (G) a wave holds 64 / G cells and each group runs its cell's line-search
 // trials in parallel; otherwise one cell per lane. Every cell runs to the end
 // of its Newton loop.
+// The 5- and 6-material tails carry the longest chains (iteration caps of
+// 640 and 768) and run one wave per SIMD so their state stays in registers;
+// the others run two.
+#ifndef SGM_TAIL_WAVES_PER_EU_LARGE
+#define SGM_TAIL_WAVES_PER_EU_LARGE 1
+#endif
 template <int NM, int METHOD>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
-                                   SGM_BIN_WAVES_PER_EU))) pte_tail_kernel(PteArgs a,
+__attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
+                                   NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2))) pte_tail_kernel(PteArgs a,
                                                                       MixParamsDev P,
                                                                       ParkArgs pk) {
   constexpr int G = TailGroup(METHOD, NM);

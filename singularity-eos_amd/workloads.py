@@ -118,6 +118,48 @@ def uneven(ctx, n, seed=SEED_BASE + 2, pool=(0, 1, 2, 3, 4, 5)):
     return a
 
 
+EV2K = 1.160451930280894026e4  # kelvin per eV (singularity-eos/eos/get_sg_eos.hpp:68)
+
+
+def driver_cells(ctx, n, seed=SEED_BASE + 3, pure_every=7, cutoff_trace=1e-9):
+    """Inputs of the get_sg_eos closure driver from the uneven workload's cells:
+    per cell the specific volume 1 / sum(rho_m vfrac_m), the mass-weighted
+    specific energy, an energy-weighted temperature guess (eV) and the
+    volume-weighted pressure; per material (six slots = the six pool materials,
+    column-major [n x 6]) the mass fractions, zero for materials not in the
+    cell. Every `pure_every`-th cell keeps only its first material; every
+    5th cell gets a trace material (mass fraction `cutoff_trace`) of a pool
+    material it lacks, which a cutoff above that excludes from the closure.
+    Returns a dict of numpy arrays plus `eos_offsets` (1-based pool indices)."""
+    a = uneven(ctx, n, seed=seed)
+    off = a["offsets"]
+    nmat = 6
+    fm = np.zeros((n, nmat))
+    spvol = np.zeros(n)
+    sie = np.zeros(n)
+    temp = np.zeros(n)
+    press = np.zeros(n)
+    for c in range(n):
+        k0, k1 = int(off[c]), int(off[c + 1])
+        if pure_every and c % pure_every == 0:
+            k1 = k0 + 1
+        mats = a["mat_ids"][k0:k1].astype(int)
+        rb = a["rho"][k0:k1] * a["vfrac"][k0:k1]
+        al = a["vfrac"][k0:k1] / a["vfrac"][k0:k1].sum()
+        fm[c, mats] = rb / rb.sum()
+        spvol[c] = al.sum() / (a["rho"][k0:k1] * al).sum()
+        sie[c] = (fm[c, mats] * a["sie"][k0:k1]).sum()
+        temp[c] = (fm[c, mats] * a["temp"][k0:k1]).sum() / EV2K
+        press[c] = abs((al * a["press"][k0:k1]).sum()) + 1e5
+        if c % 5 == 0:
+            missing = [m for m in range(nmat) if fm[c, m] == 0]
+            if missing:
+                fm[c, missing[c % len(missing)]] = cutoff_trace
+    return dict(frac_mass=np.ascontiguousarray(fm.T).ravel(), spvol=spvol, sie=sie, temp=temp,
+                press=press, eos_offsets=np.arange(1, nmat + 1, dtype=np.int32), ncell=n,
+                nmat=nmat)
+
+
 def table_queries(n, table, seed=SEED_BASE + 1):
     """log10 rho ~ U(-3, 2), log10 T ~ U(2, 6) (true log-uniform) — drawn with
     numpy's PCG64 from the seed (query points only; any seeded draw works)."""
