@@ -156,7 +156,8 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--copy-cal", action="store_true",
-                    help="end with one 1 GiB device copy (scale for FETCH/WRITE counters)")
+                    help="end with one elementwise kernel that reads 1 GiB and writes 1 GiB "
+                         "(torch.mul; the scale for FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the closure shards by cell "
@@ -329,6 +330,37 @@ def main():
         torch.cuda.synchronize()
         vms = e0.elapsed_time(e1)
         extras["vector_P_rho_sie_GBs"] = nq * 24 / (vms * 1e-3) / 1e9
+        # the get_sg_eos closure driver (rho-e input) over 2^20 cells built from
+        # the same workload: every call includes its prologue/scan/fill/epilogue
+        # passes, its temporary allocations and the PTE of the mixed cells
+        dc = wl.driver_cells(ctx, n)
+        nm_d = dc["nmat"]
+        drv_in = {k: torch.tensor(dc[k], device=dev) for k in ("spvol", "sie", "temp", "press",
+                                                                "frac_mass")}
+        drv = {k: torch.zeros(n, dtype=torch.float64, device=dev)
+               for k in ("pmax", "vol", "bmod", "dpde", "cv")}
+        drv["vol"].fill_(1.0)
+        for k in ("frac_vol", "frac_ie", "frac_bmod", "frac_dpde", "frac_cv"):
+            drv[k] = torch.zeros(n * nm_d, dtype=torch.float64, device=dev)
+        offs = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
+
+        def driver_call():
+            for k, v in drv_in.items():
+                drv[k] = v.clone()
+            rc = sgm.get_sg_eos(ctx, 1, dc["eos_offsets"].tolist(), offs, drv, 1e-6,
+                                params=params, stream=stream)
+            assert rc == 0, rc
+
+        driver_call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            driver_call()
+        torch.cuda.synchronize()
+        ddt = (time.perf_counter() - t0) / reps
+        extras["get_sg_eos_rho_e_cells_per_s"] = n / ddt
+        log(f"[bench] get_sg_eos (rho-e) driver: {ddt * 1e3:.1f} ms per call of {n} cells")
 
     if not args.no_serve:
         extras["serving"] = serve_bench(torch, ctx, a, params, args.serve_requests)
@@ -425,7 +457,8 @@ def main():
     if args.copy_cal:
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 2.0, out=dst)  # reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
     plan.close()
     ctx.close()

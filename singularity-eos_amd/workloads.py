@@ -124,37 +124,43 @@ EV2K = 1.160451930280894026e4  # kelvin per eV (singularity-eos/eos/get_sg_eos.h
 def driver_cells(ctx, n, seed=SEED_BASE + 3, pure_every=7, cutoff_trace=1e-9):
     """Inputs of the get_sg_eos closure driver from the uneven workload's cells:
     per cell the specific volume 1 / sum(rho_m vfrac_m), the mass-weighted
-    specific energy, an energy-weighted temperature guess (eV) and the
-    volume-weighted pressure; per material (six slots = the six pool materials,
-    column-major [n x 6]) the mass fractions, zero for materials not in the
-    cell. Every `pure_every`-th cell keeps only its first material; every
-    5th cell gets a trace material (mass fraction `cutoff_trace`) of a pool
-    material it lacks, which a cutoff above that excludes from the closure.
-    Returns a dict of numpy arrays plus `eos_offsets` (1-based pool indices)."""
+    specific energy, a mass-weighted temperature guess (eV) and the
+    volume-weighted pressure (absolute value + 1e5); per material (six slots =
+    the six pool materials, column-major [n x 6]) the mass fractions, zero for
+    materials not in the cell. Every `pure_every`-th cell keeps only its first
+    material; every 5th cell gets a trace material (mass fraction
+    `cutoff_trace`) of a pool material it lacks, which a cutoff above that
+    excludes from the closure. Returns a dict of numpy arrays plus
+    `eos_offsets` (1-based pool indices)."""
     a = uneven(ctx, n, seed=seed)
-    off = a["offsets"]
+    off = a["offsets"].astype(np.int64)
     nmat = 6
+    cell = np.repeat(np.arange(n), np.diff(off))
+    keep = np.ones(len(cell), dtype=bool)
+    if pure_every:
+        pure = np.zeros(n, dtype=bool)
+        pure[::pure_every] = True
+        first = np.zeros(len(cell), dtype=bool)
+        first[off[:-1]] = True
+        keep = ~pure[cell] | first
+    c, mats = cell[keep], a["mat_ids"][keep].astype(np.int64)
+    rho, vf = a["rho"][keep], a["vfrac"][keep]
+    vsum = np.bincount(c, weights=vf, minlength=n)
+    al = vf / vsum[c]
+    rb = rho * vf
+    rbs = np.bincount(c, weights=rb, minlength=n)
+    x = rb / rbs[c]
     fm = np.zeros((n, nmat))
-    spvol = np.zeros(n)
-    sie = np.zeros(n)
-    temp = np.zeros(n)
-    press = np.zeros(n)
-    for c in range(n):
-        k0, k1 = int(off[c]), int(off[c + 1])
-        if pure_every and c % pure_every == 0:
-            k1 = k0 + 1
-        mats = a["mat_ids"][k0:k1].astype(int)
-        rb = a["rho"][k0:k1] * a["vfrac"][k0:k1]
-        al = a["vfrac"][k0:k1] / a["vfrac"][k0:k1].sum()
-        fm[c, mats] = rb / rb.sum()
-        spvol[c] = al.sum() / (a["rho"][k0:k1] * al).sum()
-        sie[c] = (fm[c, mats] * a["sie"][k0:k1]).sum()
-        temp[c] = (fm[c, mats] * a["temp"][k0:k1]).sum() / EV2K
-        press[c] = abs((al * a["press"][k0:k1]).sum()) + 1e5
-        if c % 5 == 0:
-            missing = [m for m in range(nmat) if fm[c, m] == 0]
-            if missing:
-                fm[c, missing[c % len(missing)]] = cutoff_trace
+    fm[c, mats] = x
+    spvol = 1.0 / np.bincount(c, weights=rho * al, minlength=n)
+    sie = np.bincount(c, weights=x * a["sie"][keep], minlength=n)
+    temp = np.bincount(c, weights=x * a["temp"][keep], minlength=n) / EV2K
+    press = np.abs(np.bincount(c, weights=al * a["press"][keep], minlength=n)) + 1e5
+    tc = np.arange(0, n, 5)
+    for k in range(nmat):  # the trace: the first pool material the cell lacks
+        lack = tc[fm[tc, k] == 0]
+        has_trace = (fm[lack] == cutoff_trace).any(axis=1)
+        fm[lack[~has_trace], k] = cutoff_trace
     return dict(frac_mass=np.ascontiguousarray(fm.T).ravel(), spvol=spvol, sie=sie, temp=temp,
                 press=press, eos_offsets=np.arange(1, nmat + 1, dtype=np.int32), ncell=n,
                 nmat=nmat)
