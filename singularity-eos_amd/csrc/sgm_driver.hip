@@ -10,9 +10,12 @@
 //     (cell, material) entries;
 //  3. fill: the PTE inputs of every entry (densities at the cell's specific
 //     volume, the mode's temperature / pressure / energy guesses, zeroed
-//     lambda caches) and the list of cells with two or more materials;
-//  4. the plain PTE list kernel (RhoT, FixedT or FixedP by mode) over that
-//     list, then the epilogue: the single-material paths, the P-T mode's
+//     lambda caches);
+//  4. the batched PTE path (RhoT, FixedT or FixedP by mode) over the cells of
+//     two or more materials: a transient plan bins them by material count
+//     and signature and launches the register-resident persistent kernels
+//     (with straggler parking) on the caller's stream; then the epilogue:
+//     the single-material paths, the P-T mode's
 //     per-material inversions, and the reference's final_functor (cell
 //     averages, per-material bulk modulus, heat capacity, Gruneisen
 //     parameter, volume and energy fractions, pmax).
@@ -52,8 +55,6 @@ struct DrvArgs {
   double *mass_sum = nullptr;
   int *eoff = nullptr;  // [ncell + 1] entry offsets
   double *cell_in = nullptr, *vtot = nullptr;
-  unsigned long long *list_range = nullptr;  // {0, count} of the multi-material list
-  int *list = nullptr;
   int *conv = nullptr;  // optional: PTE convergence per loop index
   // per entry
   int *emat = nullptr;
@@ -151,10 +152,6 @@ __global__ void fill_kernel(DrvArgs a) {
   a.cell_in[il] = a.input == IN_RHO_T ? (n > 0 ? a.etemp[o] : 0.0)
                   : a.input == IN_RHO_P ? (n > 0 ? a.epress[o] : 0.0)
                                         : a.sie[i];
-  if (n > 1) {
-    const unsigned long long k = atomicAdd(a.list_range + 1, 1ull);
-    a.list[k] = il;
-  }
 }
 
 // the single-material paths, the P-T mode's inversions and final_functor
@@ -269,11 +266,6 @@ __global__ void epilogue_kernel(DrvArgs a) {
   a.pmax[i] = a.press[i] > a.pmax[i] ? a.press[i] : a.pmax[i];
 }
 
-__global__ void list_range_kernel(unsigned long long *r) {
-  r[0] = 0;
-  r[1] = 0;
-}
-
 template <class T>
 hipError_t dalloc(T **p, size_t n) {
   return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
@@ -342,8 +334,7 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
   if (!track(dalloc(&a.npte, ncell + 1), a.npte) ||
       !track(dalloc(&a.mass_sum, ncell), a.mass_sum) ||
       !track(dalloc(&a.eoff, ncell + 1), a.eoff) || !track(dalloc(&a.cell_in, ncell), a.cell_in) ||
-      !track(dalloc(&a.vtot, ncell), a.vtot) || !track(dalloc(&a.list, ncell), a.list) ||
-      !track(dalloc(&a.list_range, 2), a.list_range))
+      !track(dalloc(&a.vtot, ncell), a.vtot))
     return cleanup(SGM_E_HIP);
   if (hipMemsetAsync(a.npte + ncell, 0, sizeof(int), s) != hipSuccess) return cleanup(SGM_E_HIP);
   prologue_kernel<<<nb, 256, 0, s>>>(a);
@@ -366,32 +357,21 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
       !track(dalloc(&a.esie, nnz), a.esie) || !track(dalloc(&a.etemp, nnz), a.etemp) ||
       !track(dalloc(&a.epress, nnz), a.epress) || !track(dalloc(&a.lam, 3L * nnz), a.lam))
     return cleanup(SGM_E_HIP);
-  list_range_kernel<<<1, 1, 0, s>>>(a.list_range);
   fill_kernel<<<nb, 256, 0, s>>>(a);
   if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
-  if (input_int != IN_P_T) {
-    PteArgs p;
-    p.d_eos = ctx->d_eos;
-    p.nreg = ctx->nmat;
-    p.ncell = ncell;
-    p.nnz = nnz;
-    p.offsets = a.eoff;
-    p.mat_ids = a.eid;
-    p.rho = a.rho;
-    p.vfrac = a.vfrac;
-    p.sie = a.esie;
-    p.temp = a.etemp;
-    p.press = a.epress;
-    p.vfrac_tot = a.vtot;
-    p.sie_tot = a.cell_in;
-    p.lambda = a.lam;
-    p.conv = a.conv;
+  if (input_int != IN_P_T && nnz > 0) {
     const int method = input_int == IN_RHO_T   ? PTE_FIXED_T
                        : input_int == IN_RHO_P ? PTE_FIXED_P
                                                : PTE_RHOT;
-    if (launch_pte_list(method, p, to_dev(params), a.list, ncell, s,
-                        reinterpret_cast<const long *>(a.list_range)) != hipSuccess)
-      return cleanup(SGM_E_HIP);
+    sgm_plan *plan = nullptr;
+    int rc = plan_create(ctx, ncell, nnz, a.eoff, a.eid, method, 2, &plan, s);
+    if (rc != SGM_OK) return cleanup(rc);
+    sgm_pte_status st{a.conv, nullptr, nullptr, nullptr};
+    rc = plan_launch(plan, a.rho, a.vfrac, a.esie, a.etemp, a.epress, a.vtot, a.cell_in, a.lam,
+                     params, st, s);
+    hipStreamSynchronize(s);  // the plan's buffers are freed below
+    sgm_plan_destroy(plan);
+    if (rc != SGM_OK) return cleanup(rc);
   }
   epilogue_kernel<<<nb, 256, 0, s>>>(a);
   if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);

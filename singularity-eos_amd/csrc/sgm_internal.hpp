@@ -190,6 +190,15 @@ int device_cu_count();
 int sync_registry(sgm_ctx *ctx);
 // MixParams as the kernels take them (NULL = the reference's defaults)
 MixParamsDev to_dev(const sgm_mix_params *p);
+// A batched plan over a CSR topology (sgm_plan_create) whose cells with fewer
+// than min_nm materials are left out of every bin, and a direct (uncaptured)
+// launch of all its bins on stream s. Used by the get_sg_eos driver, which
+// solves only its cells of two or more materials.
+int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+                int method, int min_nm, sgm_plan **out, hipStream_t s);
+int plan_launch(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                double *press, const double *vfrac_tot, const double *sie_tot, double *lambda,
+                const sgm_mix_params *params, sgm_pte_status status, hipStream_t s);
 // Gruneisen::ComputeRhoMax (eos_gruneisen.hpp:231-329), host side
 Real gruneisen_compute_rho_max(Real s1, Real s2, Real s3, Real rho0);
 } // namespace sgm

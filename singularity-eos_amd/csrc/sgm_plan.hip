@@ -96,8 +96,15 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
     if (b.nm < 2 || b.nm > 6) continue;
     std::map<unsigned long long, int> run_id;
     std::vector<long> len;
+    unsigned long long last_k = ~0ull;
+    int last_r = -1;
     for (long j = 0; j < b.n; ++j) {
       const unsigned long long k = keys[b.start + j] & ~(1ull << 55);  // signature only
+      if (k == last_k) {  // runs are contiguous in signature order
+        run_of[b.start + j] = last_r;
+        ++len[last_r];
+        continue;
+      }
       auto it = run_id.find(k);
       int r;
       if (it == run_id.end()) {
@@ -109,6 +116,8 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
       }
       run_of[b.start + j] = r;
       ++len[r];
+      last_k = k;
+      last_r = r;
     }
     for (int r = 0; r < (int)len.size(); ++r) {
       const long cap = std::min<long>(len[r], std::max<long>(64, (len[r] / 4 + 63) / 64 * 64));
@@ -305,16 +314,15 @@ hipError_t sort_cells(sgm_plan *p, const int *cost, int threshold,
 
 } // namespace
 
-extern "C" {
+namespace sgm {
 
-int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
-                    const uint8_t *mat_ids, int method, sgm_plan **out, sgm_stream_t stream) {
+int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+                int method, int min_nm, sgm_plan **out, hipStream_t s) {
   if (!ctx || !out || ncell < 1 || nnz < 1 || nnz > 2147483647L || ncell > 2147483647L ||
       !offsets || !mat_ids || method < 0 || method >= PTE_NMETHODS)
     return SGM_E_ARG;
   *out = nullptr;
   if (ctx->host_only) return SGM_E_NOGPU;
-  hipStream_t s = (hipStream_t)stream;
   auto *p = new sgm_plan();
   p->ctx = ctx;
   p->method = method;
@@ -333,14 +341,16 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
     sgm_plan_destroy(p);
     return SGM_E_HIP;
   }
-  // bins: one per material count 2..6; everything else in one generic list bin
+  // bins: one per material count 2..6; everything else in one generic list
+  // bin per count (cells with fewer than min_nm materials in none)
   long i = 0;
-  Bin other{0, -1, 0};
   while (i < ncell) {
     const int nm = (int)(keys[i] >> 56);
     long j = i;
     while (j < ncell && (int)(keys[j] >> 56) == nm) ++j;
-    if (nm >= 2 && nm <= 6) {
+    if (nm < min_nm) {
+      // not solved
+    } else if (nm >= 2 && nm <= 6) {
       Bin b{nm, i, j - i};
       p->bins.push_back(b);
     } else {
@@ -351,7 +361,6 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
     }
     i = j;
   }
-  (void)other;
   default_grids(p->bins);
   if (alloc_park(p, keys) != hipSuccess ||
       hipMalloc(&p->d_counters, sizeof(int) * std::max<size_t>(1, p->bins.size())) !=
@@ -378,6 +387,24 @@ int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
   }
   *out = p;
   return SGM_OK;
+}
+
+int plan_launch(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
+                double *press, const double *vfrac_tot, const double *sie_tot, double *lambda,
+                const sgm_mix_params *params, sgm_pte_status status, hipStream_t s) {
+  const MixParamsDev P = params_dev(params);
+  const PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, nullptr,
+                              lambda, status);
+  return launch_bins(p, a, P, s, true) == hipSuccess ? SGM_OK : SGM_E_HIP;
+}
+
+} // namespace sgm
+
+extern "C" {
+
+int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                    const uint8_t *mat_ids, int method, sgm_plan **out, sgm_stream_t stream) {
+  return plan_create(ctx, ncell, nnz, offsets, mat_ids, method, 0, out, (hipStream_t)stream);
 }
 
 int sgm_plan_describe(const sgm_plan *p, long *out, int max_bins) {
