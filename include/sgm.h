@@ -275,12 +275,15 @@ int sgm_plan_set_shape(sgm_plan *plan, int bin, int block, int cells_per_wave, i
  * only such cells, so the slowest cells neither stall the bulk of the bin nor
  * hold its compute units. Results do not change. 0 disables; default 40. */
 int sgm_plan_set_straggler_threshold(sgm_plan *plan, int iterations);
-/* one bin's kernels without the graph (per-kernel timing and profiling) */
+/* one bin's kernels without the graph (per-kernel timing and profiling).
+ * kernel_ms (may be NULL): receives the durations of the bin's persistent
+ * kernel and of its straggler tail kernel (0 without one), from HIP events
+ * recorded on `stream` around each launch; the call then synchronizes. */
 int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, double *sie,
                          double *temp, double *press, const double *vfrac_tot,
                          const double *sie_tot, const double *Tnorm, double *lambda,
                          const sgm_mix_params *params, sgm_pte_status status,
-                         sgm_stream_t stream);
+                         float *kernel_ms, sgm_stream_t stream);
 /* Reorders cells within each material count so that cells whose cost (e.g.
  * the niter status of a previous execute) reaches 4x the mean come first:
  * the slowest cells start with the bulk instead of trailing it. Results do

@@ -123,7 +123,7 @@ _SIGS = {
     "sgm_plan_execute": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(MixParams),
                               PteStatus, vp]),
     "sgm_plan_execute_bin": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                  C.POINTER(MixParams), PteStatus, vp]),
+                                  C.POINTER(MixParams), PteStatus, vp, vp]),
     "sgm_plan_describe": (ci, [vp, C.POINTER(C.c_long), ci]),
     "sgm_plan_reorder": (ci, [vp, vp, vp]),
     "sgm_plan_destroy": (ci, [vp]),
@@ -353,13 +353,19 @@ class Plan:
 
     def execute_bin(self, b, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm=None,
                     lam=None, params=None, conv=None, niter=None, resid=None, err=None,
-                    stream=0):
+                    stream=0, kernel_ms=None):
+        """One bin's kernels without the graph. kernel_ms: a list that receives
+        [persistent kernel ms, tail kernel ms] (HIP events; the call syncs)."""
         st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
-        return lib().sgm_plan_execute_bin(self.h, b, ptr(rho), ptr(vfrac), ptr(sie), ptr(temp),
-                                          ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm),
-                                          ptr(lam),
-                                          C.byref(params) if params is not None else None, st,
-                                          stream)
+        ms = (C.c_float * 2)() if kernel_ms is not None else None
+        rc = lib().sgm_plan_execute_bin(self.h, b, ptr(rho), ptr(vfrac), ptr(sie), ptr(temp),
+                                        ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm),
+                                        ptr(lam),
+                                        C.byref(params) if params is not None else None, st,
+                                        ms, stream)
+        if kernel_ms is not None:
+            kernel_ms[:] = [ms[0], ms[1]]
+        return rc
 
     def reorder(self, cost, stream=0):
         return check(lib().sgm_plan_reorder(self.h, ptr(cost), stream), "plan_reorder")

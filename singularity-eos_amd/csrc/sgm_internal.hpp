@@ -103,11 +103,15 @@ struct ParkArgs {
 // Closure formulations, by the method ids of the C ABI.
 enum PteMethod { PTE_RHOT = 0, PTE_PT = 1, PTE_FIXED_T = 2, PTE_FIXED_P = 3, PTE_NMETHODS = 4 };
 
-// Lanes per parked cell in the tail kernels: with 4+ materials (the long
-// iteration caps) a group of 8 lanes runs each cell's line-search trials side
-// by side (pte_iterate_group); PT's trials are not independent, and small
-// cells have short caps, so those keep one lane per cell.
-constexpr int TailGroup(int method, int nm) { return (method != PTE_PT && nm >= 4) ? 8 : 1; }
+// Lanes per parked cell in the tail kernels: a group of 8 lanes runs each
+// cell's line-search trials side by side (pte_iterate_group); PT's trials
+// are not independent, so PT keeps one lane per cell.
+#ifndef SGM_TAIL_GROUP_MIN_NM
+#define SGM_TAIL_GROUP_MIN_NM 2
+#endif
+constexpr int TailGroup(int method, int nm) {
+  return (method != PTE_PT && nm >= SGM_TAIL_GROUP_MIN_NM) ? 8 : 1;
+}
 
 // Per-method launchers, one translation unit pair per method
 // (sgm_pte_plain_<name>.hip: plain + list kernels; sgm_pte_bins_<name>.hip:

@@ -49,8 +49,8 @@ struct sgm_plan {
   int park_after = 40;           // Newton iterations before a cell is parked; 0 = never
   std::vector<Bin> bins;
   hipStream_t cap_stream = nullptr;
-  // capture-time fork streams: each bin's kernel is its own graph branch, so
-  // the bins' tails (the slowest cells) overlap instead of adding up
+  // fork streams, one per bin: each bin's tail kernel (its slowest cells)
+  // runs beside the later bins' kernels instead of after them
   std::vector<hipStream_t> aux;
   std::vector<hipEvent_t> ev;
   hipGraph_t graph = nullptr;
@@ -61,12 +61,12 @@ struct sgm_plan {
 
 namespace {
 
-// Default persistent grids. The bin and tail kernels run two waves per SIMD,
-// so 8 waves per CU fill the GPU. Each bin gets 4 waves per CU: the bins'
-// bulk phases overlap, and SIMDs stay free for the tail kernels, which carry
-// the longest chains (measured on the uneven 2^20-cell workload). One-wave
-// blocks: a SIMD whose wave has finished takes a wave of any bin at once,
-// instead of waiting for the other waves of a multi-wave block.
+// Default persistent grids: 8 one-wave blocks per CU, two waves per SIMD,
+// the occupancy the bin kernels are built for. The bins' persistent kernels
+// run one after another, each on the whole GPU, while the tail kernels of
+// the bins before them finish their parked cells beside it (launch_bins).
+// One-wave blocks: a SIMD whose wave has finished takes the next wave at
+// once, instead of waiting for the other waves of a multi-wave block.
 void default_grids(std::vector<Bin> &bins) {
   const int ncu = device_cu_count();
   for (Bin &b : bins) {
@@ -74,7 +74,7 @@ void default_grids(std::vector<Bin> &bins) {
     b.block = 64;
     b.cpw = 64;
     const long need = (b.n + b.block - 1) / b.block;  // no more waves than cells fill
-    b.grid = (int)std::max<long>(1, std::min<long>(need, 4L * ncu));
+    b.grid = (int)std::max<long>(1, std::min<long>(need, 8L * ncu));
   }
 }
 
@@ -196,48 +196,58 @@ hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixPara
   return launch_pte_tail(p->method, b.nm, a, P, pk, s);
 }
 
-// Graph branches, at most four (the process has four hardware queues; a
-// fifth branch would share a queue with another and wait for all of it). The
-// bins of 6, 5, 4 and 3 materials each get a branch; the remaining bins follow
-// the 6-material bin, whose bulk phase ends first (its cells' long chains run
-// in its tail kernel).
-std::vector<std::vector<size_t>> branches(const sgm_plan *p) {
-  std::vector<std::vector<size_t>> br;
-  std::vector<size_t> rest;
-  for (int nm = 6; nm >= 3; --nm)
+// Order of the bins on the launch stream: the bins whose tail kernels run
+// longest first (5, 4, 3, 6 and 2 materials on the uneven workload), so
+// their tails overlap the later bins' persistent kernels; then the list bins.
+std::vector<size_t> bin_order(const sgm_plan *p) {
+  std::vector<size_t> order;
+  for (int nm : {5, 4, 3, 6, 2})
     for (size_t i = 0; i < p->bins.size(); ++i)
-      if (p->bins[i].nm == nm) br.push_back({i});
+      if (p->bins[i].nm == nm) order.push_back(i);
   for (size_t i = 0; i < p->bins.size(); ++i)
-    if (p->bins[i].nm < 3 || p->bins[i].nm > 6) rest.push_back(i);
-  if (br.empty()) br.push_back({});
-  for (size_t i : rest) br[0].push_back(i);
-  return br;
+    if (p->bins[i].nm < 2 || p->bins[i].nm > 6) order.push_back(i);
+  return order;
 }
 
-// Records the counter reset and every bin on stream s. With fork streams
-// (graph capture) each branch runs on its own stream between a fork and a
-// join event.
+// Records the counter reset and every bin on stream s: the persistent bin
+// kernels one after another on s; with fork streams (graph capture, or a
+// direct launch that may overlap) each bin's tail kernel forks off to a
+// stream of its own after its bin kernel and joins s at the end, so the
+// tails run beside the later bins. Without forks everything stays on s.
 hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hipStream_t s,
                        bool fork) {
   hipError_t e = reset_counters(p, s);
   if (e != hipSuccess) return e;
-  const std::vector<std::vector<size_t>> br = branches(p);
-  const size_t nbr = br.size();
-  if (!fork || nbr < 2 || p->aux.size() < nbr || p->ev.size() < nbr + 1) {
-    for (const auto &b : br)
-      for (size_t i : b)
-        if ((e = launch_one_bin(p, i, a, P, s)) != hipSuccess) return e;
-    return hipSuccess;
+  const std::vector<size_t> order = bin_order(p);
+  const size_t nb = p->bins.size();
+  fork = fork && p->aux.size() >= nb && p->ev.size() >= 2 * nb;
+  std::vector<size_t> joins;
+  for (size_t i : order) {
+    const Bin &b = p->bins[i];
+    const int *cells = p->d_sorted + b.start;
+    if (b.nm < 2 || b.nm > 6) {
+      if ((e = launch_pte_list(p->method, a, P, cells, b.n, s)) != hipSuccess) return e;
+      continue;
+    }
+    ParkArgs pk = b.park;
+    pk.after = pk.rec ? p->park_after : 0;
+    if ((e = launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
+                            b.cpw, b.grid, s, nullptr, &pk)) != hipSuccess)
+      return e;
+    if (pk.after <= 0) continue;
+    if (!fork) {
+      if ((e = launch_pte_tail(p->method, b.nm, a, P, pk, s)) != hipSuccess) return e;
+      continue;
+    }
+    if ((e = hipEventRecord(p->ev[2 * i], s)) != hipSuccess ||
+        (e = hipStreamWaitEvent(p->aux[i], p->ev[2 * i], 0)) != hipSuccess ||
+        (e = launch_pte_tail(p->method, b.nm, a, P, pk, p->aux[i])) != hipSuccess ||
+        (e = hipEventRecord(p->ev[2 * i + 1], p->aux[i])) != hipSuccess)
+      return e;
+    joins.push_back(i);
   }
-  if ((e = hipEventRecord(p->ev[0], s)) != hipSuccess) return e;
-  for (size_t k = 0; k < nbr; ++k) {
-    if ((e = hipStreamWaitEvent(p->aux[k], p->ev[0], 0)) != hipSuccess) return e;
-    for (size_t i : br[k])
-      if ((e = launch_one_bin(p, i, a, P, p->aux[k])) != hipSuccess) return e;
-    if ((e = hipEventRecord(p->ev[k + 1], p->aux[k])) != hipSuccess) return e;
-  }
-  for (size_t k = 0; k < nbr; ++k)
-    if ((e = hipStreamWaitEvent(s, p->ev[k + 1], 0)) != hipSuccess) return e;
+  for (size_t i : joins)
+    if ((e = hipStreamWaitEvent(s, p->ev[2 * i + 1], 0)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -377,7 +387,7 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
     }
     p->aux.push_back(q);
   }
-  for (size_t i = 0; i <= p->bins.size(); ++i) {
+  for (size_t i = 0; i < 2 * p->bins.size(); ++i) {  // per bin: tail fork and join
     hipEvent_t ev = nullptr;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
       sgm_plan_destroy(p);
@@ -515,7 +525,7 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
       for (int cpw : cpws)
         for (int sc : scales) {
           const int wpb = bl / 64;
-          const int grid = std::max(1, std::min(4 * ncu, share_waves * sc / 2 / wpb));
+          const int grid = std::max(1, std::min(8 * ncu, share_waves * sc / 2 / wpb));
           float tot = 0;
           for (int r = 0; r < reps && rc == SGM_OK; ++r) {
             if (hipMemcpyAsync(w[0], rho, nb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
@@ -664,7 +674,7 @@ int sgm_plan_set_straggler_threshold(sgm_plan *p, int iterations) {
 int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, double *sie,
                          double *temp, double *press, const double *vfrac_tot,
                          const double *sie_tot, const double *Tnorm, double *lambda,
-                         const sgm_mix_params *params, sgm_pte_status status,
+                         const sgm_mix_params *params, sgm_pte_status status, float *kernel_ms,
                          sgm_stream_t stream) {
   if (!p || bin < 0 || bin >= (int)p->bins.size() || !rho || !vfrac || !sie || !temp ||
       !press || !vfrac_tot || !sie_tot)
@@ -675,7 +685,33 @@ int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, doubl
                         status);
   hipError_t e = reset_counters(p, s);
   if (e != hipSuccess) return SGM_E_HIP;
-  e = launch_one_bin(p, (size_t)bin, a, P, s);
+  if (!kernel_ms) {
+    e = launch_one_bin(p, (size_t)bin, a, P, s);
+    return e == hipSuccess ? SGM_OK : SGM_E_HIP;
+  }
+  // the same launches as launch_one_bin, with an event between them
+  const Bin &b = p->bins[(size_t)bin];
+  const int *cells = p->d_sorted + b.start;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  for (hipEvent_t &x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+  ParkArgs pk = b.park;
+  pk.after = (b.nm >= 2 && b.nm <= 6 && pk.rec) ? p->park_after : 0;
+  if (e == hipSuccess)
+    e = (b.nm < 2 || b.nm > 6)
+            ? launch_pte_list(p->method, a, P, cells, b.n, s)
+            : launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + bin, b.block,
+                             b.cpw, b.grid, s, nullptr, &pk);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+  if (e == hipSuccess && pk.after > 0) e = launch_pte_tail(p->method, b.nm, a, P, pk, s);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&kernel_ms[0], ev[0], ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&kernel_ms[1], ev[1], ev[2]);
+  if (e == hipSuccess && pk.after <= 0) kernel_ms[1] = 0.0f;
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
   return e == hipSuccess ? SGM_OK : SGM_E_HIP;
 }
 

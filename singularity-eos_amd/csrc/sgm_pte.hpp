@@ -1286,6 +1286,45 @@ __device__ __forceinline__ void group_take(S &s, DriverState &d, CellStatus &st,
   VisitDriver(d, st, take);
 }
 
+// Fixed points of the Newton loop. A cell that stalls at the roundoff floor
+// often maps its whole iteration-live state onto itself bit for bit; every
+// further pass of the loop body then repeats that state until the iteration
+// cap, where the last pass also applies the cap's slow-convergence test.
+// fixed_point_probe compares the cell's state (every word Visit and
+// VisitDriver see, the iteration counters aside) with the copy in w, then
+// stores the current state there; true when the two are identical. A loop
+// that finds its state unchanged by one pass may skip to the last pass: the
+// skipped passes would reproduce the same state, so the state, counters and
+// status after the last pass are those of the full loop.
+struct FixedPointProbe {
+  double *w;
+  int k;
+  bool same;
+  template <class T>
+  SGM_HD void operator()(T &x) {
+    const double v = (double)x;
+    same = same && __builtin_bit_cast(unsigned long long, v) ==
+                       __builtin_bit_cast(unsigned long long, w[k]);
+    w[k++] = v;
+  }
+};
+template <class Sys>
+SGM_HD bool fixed_point_probe(Sys &s, DriverState &d, CellStatus &st, double *w) {
+  const int n0 = d.niter, s0 = st.niter;
+  d.niter = 0;
+  st.niter = 0;
+  FixedPointProbe f{w, 0, true};
+  s.Visit(f);
+  VisitDriver(d, st, f);
+  d.niter = n0;
+  st.niter = s0;
+  return f.same;
+}
+// after a pass whose state matched the probe copy: resume at the last pass
+SGM_HD inline void skip_to_last_pass(DriverState &d) {
+  if (d.niter < d.max_iter - 1) d.niter = d.max_iter - 1;
+}
+
 // Whether one TestUpdate's result depends only on the state before the line
 // search and its own scale (so trial steps can run on separate lanes):
 // RhoT, FixedT and FixedP evaluate each material at the trial state; PT's

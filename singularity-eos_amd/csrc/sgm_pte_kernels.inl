@@ -342,18 +342,25 @@ __attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
   SysOf<NM, METHOD> s;
   DriverState d;
   CellStatus st;
-  ParkLoad(s, d, st, pk.rec + slot * pk.stride);
+  double *rec = pk.rec + slot * pk.stride;
+  ParkLoad(s, d, st, rec);
   s.P = P;
+  // the record, free once loaded, keeps the state before each pass for the
+  // fixed-point probe (written by the group's first lane only)
   bool grouped = false;
   if constexpr (G > 1) {
     grouped = P.line_search_max_iter <= G;
     if (grouped)
       while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
+        int same = 0;
+        if (g == 0) same = fixed_point_probe(s, d, st, rec) ? 1 : 0;
+        if (__shfl(same, base, 64)) skip_to_last_pass(d);
       }
   }
   if (!grouped) {
     if (g != 0) return;
     while (!pte_iterate<NM>(s, d, st)) {
+      if (fixed_point_probe(s, d, st, rec)) skip_to_last_pass(d);
     }
   }
   pte_end<NM>(s, d, st);
