@@ -273,7 +273,7 @@ int sgm_plan_set_shape(sgm_plan *plan, int bin, int block, int cells_per_wave, i
  * still iterating after `iterations` Newton iterations are parked (their
  * solver state saved) and finished by a per-bin tail kernel whose waves hold
  * only such cells, so the slowest cells neither stall the bulk of the bin nor
- * hold its compute units. Results do not change. 0 disables; default 40. */
+ * hold its compute units. Results do not change. 0 disables; default 20. */
 int sgm_plan_set_straggler_threshold(sgm_plan *plan, int iterations);
 /* one bin's kernels without the graph (per-kernel timing and profiling).
  * kernel_ms (may be NULL): receives the durations of the bin's persistent

@@ -98,6 +98,11 @@ struct ParkArgs {
   const int *wave_run = nullptr;    // per tail wave: its run
   const int *wave_first = nullptr;  // per tail wave: its first record within the run
   int nwaves = 0;
+  // the tail's second level: a cell still iterating `after2` iterations into
+  // the first (one lane per cell) is saved again in its record for the
+  // second, where TailGroup lanes share each cell
+  int after2 = 0;                   // 0: one level
+  int *l2_count = nullptr;          // cells saved for the second level
 };
 
 // Closure formulations, by the method ids of the C ABI.

@@ -46,7 +46,8 @@ struct sgm_plan {
   int *d_run_of = nullptr, *d_run_base = nullptr, *d_run_cap = nullptr;
   int *d_run_count = nullptr, *d_wave_run = nullptr, *d_wave_first = nullptr;
   int nruns = 0;
-  int park_after = 40;           // Newton iterations before a cell is parked; 0 = never
+  int *d_l2_counts = nullptr;    // per bin: cells saved for the tail's second level
+  int park_after = 20;           // Newton iterations before a cell is parked; 0 = never
   std::vector<Bin> bins;
   hipStream_t cap_stream = nullptr;
   // fork streams, one per bin: each bin's tail kernel (its slowest cells)
@@ -80,11 +81,12 @@ void default_grids(std::vector<Bin> &bins) {
 
 // Straggler records for every register-resident bin, from the sort keys of
 // the bin's positions: the cells of one signature (a "run"; contiguous in
-// signature order, scattered after a cost reorder) of count L get
-// min(L, max(64, L/4 rounded up to 64)) records of ParkWords FP64 words and
-// one tail wave per 64 / TailGroup records. (About 1-3 % of the cells of the
-// uneven workload run past 40 iterations; a run whose records are used up
-// keeps its further stragglers in the bulk kernel.)
+// signature order, scattered after a cost reorder) get one record of
+// ParkWords FP64 words each, and one tail wave slot per 64 / TailGroup
+// records. Every cell can park: a straggler that found no record would run
+// to the end of its Newton loop (up to 128 x nm iterations) in the bin
+// kernel and hold its wave there. The records cost (11 nm + 25) x 8 bytes
+// per cell, 0.5 GB for the uneven 2^20-cell workload.
 hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) {
   std::vector<int> run_of(p->ncell, 0), run_base, run_cap, wave_run, wave_first;
   std::vector<size_t> run0(p->bins.size()), wave0(p->bins.size());
@@ -120,7 +122,7 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
       last_r = r;
     }
     for (int r = 0; r < (int)len.size(); ++r) {
-      const long cap = std::min<long>(len[r], std::max<long>(64, (len[r] / 4 + 63) / 64 * 64));
+      const long cap = len[r];
       run_base.push_back((int)nrec[bi]);
       run_cap.push_back((int)cap);
       const int per_wave = 64 / TailGroup(p->method, b.nm);
@@ -145,6 +147,8 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
   if (e == hipSuccess) e = up(&p->d_run_count, run_cap);  // contents reset per execution
   if (e == hipSuccess) e = up(&p->d_wave_run, wave_run);
   if (e == hipSuccess) e = up(&p->d_wave_first, wave_first);
+  if (e == hipSuccess)
+    e = hipMalloc(&p->d_l2_counts, sizeof(int) * std::max<size_t>(1, p->bins.size()));
   for (size_t bi = 0; bi < p->bins.size() && e == hipSuccess; ++bi) {
     Bin &b = p->bins[bi];
     if (b.nm < 2 || b.nm > 6 || nrec[bi] == 0) continue;
@@ -152,6 +156,8 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
     k.stride = ParkWords(b.nm, b.nm + 1);
     e = hipMalloc(&k.rec, sizeof(double) * k.stride * (size_t)nrec[bi]);
     if (e == hipSuccess) e = hipMalloc(&k.cell_of, sizeof(int) * nrec[bi]);
+    k.l2_count = p->d_l2_counts + bi;
+    k.after2 = 4;
     k.run_of = p->d_run_of + b.start;
     k.run_base = p->d_run_base + run0[bi];
     k.run_cap = p->d_run_cap + run0[bi];
@@ -168,7 +174,7 @@ void free_park(sgm_plan *p) {
     for (void *q : {(void *)b.park.rec, (void *)b.park.cell_of})
       if (q) (void)hipFree(q);
   for (int *q : {p->d_run_of, p->d_run_base, p->d_run_cap, p->d_run_count, p->d_wave_run,
-                 p->d_wave_first})
+                 p->d_wave_first, p->d_l2_counts})
     if (q) (void)hipFree(q);
 }
 
@@ -178,6 +184,8 @@ hipError_t reset_counters(sgm_plan *p, hipStream_t s) {
   hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
   if (e == hipSuccess && p->park_after > 0 && p->nruns > 0)
     e = launch_zero_ints(p->d_run_count, p->nruns, s);
+  if (e == hipSuccess && p->park_after > 0 && p->d_l2_counts)
+    e = launch_zero_ints(p->d_l2_counts, (int)p->bins.size(), s);
   return e;
 }
 
@@ -197,11 +205,12 @@ hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixPara
 }
 
 // Order of the bins on the launch stream: the bins whose tail kernels run
-// longest first (5, 4, 3, 6 and 2 materials on the uneven workload), so
-// their tails overlap the later bins' persistent kernels; then the list bins.
+// longest first (5, 4, 6, 3 and 2 materials on the uneven workload: 17, 17,
+// 15, 11 and 4 ms), so their tails overlap the later bins' persistent
+// kernels; then the list bins.
 std::vector<size_t> bin_order(const sgm_plan *p) {
   std::vector<size_t> order;
-  for (int nm : {5, 4, 3, 6, 2})
+  for (int nm : {5, 4, 6, 3, 2})
     for (size_t i = 0; i < p->bins.size(); ++i)
       if (p->bins[i].nm == nm) order.push_back(i);
   for (size_t i = 0; i < p->bins.size(); ++i)

@@ -234,7 +234,7 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
   CellStatus st;
   long c = -1, jpos = -1;
   int off = 0;
-  bool park_ok = pk.after > 0;  // this lane may still park its cell
+  bool park_ok = false;  // this lane may still park its cell (one attempt per cell)
   bool active = false;  // this lane holds a cell that is still iterating
   bool done = false;    // the bin is exhausted for this wave
   long rbase = 0, rend = 0;  // wave-uniform reserve [rbase, rend) of bin positions
@@ -274,6 +274,7 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
             pte_begin<NM>(s, Tin, d, st);
             if (a.guess_T) start_from_guess<NM>(s, Tin, d, st, lds, a, P, c, off);
             active = true;
+            park_ok = pk.after > 0;
           } else if (c >= 0 && c < a.ncell) {
             mark_bad(a, c);
           }
@@ -306,11 +307,18 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
   }
 }
 
-// Resumes the parked cells of one bin: one 64-lane block per tail wave, each
-// wave on records of a single signature run. With TailGroup lanes per cell
-// (G) a wave holds 64 / G cells and each group runs its cell's line-search
-// trials in parallel; otherwise one cell per lane. Every cell runs to the end
-// of its Newton loop.
+// Resumes the parked cells of one bin in two levels; a persistent grid of
+// one-wave blocks walks the bin's tail wave slots (each on records of a
+// single signature run) in both. Level 1: one lane per cell; a cell that
+// reaches a fixed point of the Newton loop skips to its last pass, and a
+// cell still iterating pk.after2 iterations later is saved back into its
+// record for level 2 (finished cells clear their record's cell id). Level 2:
+// TailGroup lanes (G) per remaining cell, which run its line-search trials
+// side by side (pte_iterate_group), when those cells fit the grid at once
+// (else one lane each). Every cell runs to the end of its Newton loop. The
+// many cells that stall at a fixed point or finish soon keep the throughput
+// of one lane each; the few that wander for hundreds of iterations get the
+// shorter iterations of a group.
 // The 5- and 6-material tails carry the longest chains (iteration caps of
 // 640 and 768) and run one wave per SIMD so their state stays in registers;
 // the others run two.
@@ -322,49 +330,71 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
                                    NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2))) pte_tail_kernel(PteArgs a,
                                                                       MixParamsDev P,
-                                                                      ParkArgs pk) {
+                                                                      ParkArgs pk,
+                                                                      int level) {
   constexpr int G = TailGroup(METHOD, NM);
-  const int w = blockIdx.x;
-  if (w >= pk.nwaves) return;
-  const int r = pk.wave_run[w];
-  const int nr = min(pk.run_count[r], pk.run_cap[r]);
-  const int first = pk.wave_first[w];
-  if (first >= nr) return;  // the whole wave is empty
+  constexpr int per_slot = 64 / G;  // records per wave slot
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
   const int lane = (int)threadIdx.x;
-  const int g = lane % G, base = lane - g;
-  const int i = first + lane / G;
-  if (i >= nr) return;  // whole groups leave together
-  const long slot = (long)pk.run_base[r] + i;
-  const long c = pk.cell_of[slot];
-  const int off = a.offsets[c];
-  SysOf<NM, METHOD> s;
-  DriverState d;
-  CellStatus st;
-  double *rec = pk.rec + slot * pk.stride;
-  ParkLoad(s, d, st, rec);
-  s.P = P;
-  // the record, free once loaded, keeps the state before each pass for the
-  // fixed-point probe (written by the group's first lane only)
-  bool grouped = false;
-  if constexpr (G > 1) {
-    grouped = P.line_search_max_iter <= G;
-    if (grouped)
-      while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
-        int same = 0;
-        if (g == 0) same = fixed_point_probe(s, d, st, rec) ? 1 : 0;
-        if (__shfl(same, base, 64)) skip_to_last_pass(d);
-      }
+  const bool two_levels = G > 1 && pk.after2 > 0 && pk.l2_count && P.line_search_max_iter <= G;
+  bool grp = false;
+  if (level == 2) {
+    if (!two_levels) return;
+    grp = (long)(*pk.l2_count) * G <= 64L * (long)gridDim.x;
   }
-  if (!grouped) {
-    if (g != 0) return;
-    while (!pte_iterate<NM>(s, d, st)) {
-      if (fixed_point_probe(s, d, st, rec)) skip_to_last_pass(d);
+  const int lpc = grp ? G : 1;  // lanes per cell
+  const int spb = grp ? 1 : G;  // wave slots per block step
+  const int g = lane % lpc, base = lane - g;
+  for (long W = blockIdx.x; W * spb < pk.nwaves; W += gridDim.x) {
+    const long w = grp ? W : W * G + lane / per_slot;
+    if (w >= pk.nwaves) continue;
+    const int r = pk.wave_run[w];
+    const int nr = min(pk.run_count[r], pk.run_cap[r]);
+    const int i = pk.wave_first[w] + (grp ? lane / G : lane % per_slot);
+    if (i >= nr) continue;  // in a group, the whole group skips
+    const long slot = (long)pk.run_base[r] + i;
+    const long c = pk.cell_of[slot];
+    if (c < 0) continue;  // finished in level 1
+    const int off = a.offsets[c];
+    SysOf<NM, METHOD> s;
+    DriverState d;
+    CellStatus st;
+    // the record, free once loaded, keeps the state before each pass for the
+    // fixed-point probe (in a group, written by its first lane only)
+    double *rec = pk.rec + slot * pk.stride;
+    ParkLoad(s, d, st, rec);
+    s.P = P;
+    bool moved = false;
+    if constexpr (G > 1) {
+      if (grp) {
+        while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
+          int same = 0;
+          if (g == 0) same = fixed_point_probe(s, d, st, rec) ? 1 : 0;
+          if (__shfl(same, base, 64)) skip_to_last_pass(d);
+        }
+      }
+    }
+    if (!grp) {
+      const bool can_move = level == 1 && two_levels;
+      const int stop = d.niter + pk.after2;
+      while (!pte_iterate<NM>(s, d, st)) {
+        if (fixed_point_probe(s, d, st, rec)) skip_to_last_pass(d);
+        if (can_move && d.niter >= stop) {
+          ParkSave(s, d, st, rec);
+          atomicAdd(pk.l2_count, 1);
+          moved = true;
+          break;
+        }
+      }
+    }
+    if (moved) continue;
+    pte_end<NM>(s, d, st);
+    if (g == 0) {
+      store_cell<NM>(s, st, a, c, off);
+      if (level == 1 && two_levels) pk.cell_of[slot] = -1;
     }
   }
-  pte_end<NM>(s, d, st);
-  if (g == 0) store_cell<NM>(s, st, a, c, off);
 }
 
 template <int METHOD>
@@ -410,16 +440,28 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   if (pk.nwaves <= 0) return hipSuccess;
   if (pk.stride < ParkWords(nm, nm + 1)) return hipErrorInvalidValue;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
-  const int g = pk.nwaves;
-  switch (nm) {
-  case 2: pte_tail_kernel<2, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
-  case 3: pte_tail_kernel<3, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
-  case 4: pte_tail_kernel<4, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
-  case 5: pte_tail_kernel<5, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
-  case 6: pte_tail_kernel<6, METHOD><<<g, 64, lds, s>>>(a, P, pk); break;
-  default: return hipErrorInvalidValue;
+  // persistent: at most as many waves as the GPU holds at the kernel's
+  // occupancy (one wave per SIMD from 5 materials, else two)
+  const int ncu = device_cu_count();
+  const int cap = (nm >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2) * 4 * ncu;
+  const int g1 = (pk.nwaves + TailGroup(METHOD, nm) - 1) / TailGroup(METHOD, nm);
+  const int g = g1 < cap ? g1 : cap;
+  const bool l2 = TailGroup(METHOD, nm) > 1 && pk.after2 > 0 && pk.l2_count &&
+                  P.line_search_max_iter <= TailGroup(METHOD, nm);
+  for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
+    const int gl = level == 1 ? g : (pk.nwaves < cap ? pk.nwaves : cap);
+    switch (nm) {
+    case 2: pte_tail_kernel<2, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    case 3: pte_tail_kernel<3, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    case 4: pte_tail_kernel<4, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    case 5: pte_tail_kernel<5, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    case 6: pte_tail_kernel<6, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    default: return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 } // namespace kern
