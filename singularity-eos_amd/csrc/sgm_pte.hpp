@@ -560,6 +560,29 @@ struct PteRhoT : PteBase<NM, NM> {
     conv = cp && cu;
     ok = ap && au;
   }
+  // One material's finite-difference derivatives of the Jacobian (the body
+  // of the reference's material loop, :1042-1080): two fused e/P
+  // evaluations, which also update that material's lambda cache.
+  SGM_HD void MaterialDerivs(const EosDev &e, Lam &l, Real vf, Real rb, Real rh, Real pr,
+                             Real um, Real Tcur, Real Tpert, Real dT, Real &dpdv, Real &dedv,
+                             Real &dpdT, Real &dedT) const {
+    const Real dv = (vf < 0.5 ? 1.0 : -1.0) * vf * P.derivative_eps;
+    const Real vf_pert = vf + dv;
+    const Real rho_pert = robust::ratio(rb, vf_pert);
+    Real P_T;
+    Real e_pert = eos_e_rT_with_P(e, rho_pert, Tcur, l, P_T);
+    Real p_pert = robust::ratio(PressureFromPreferredE(e, l, rho_pert, e_pert, P_T), uscale);
+    dpdv = smin(robust::ratio((p_pert - pr), dv), -fabs(P.pte_max_dpdv));
+    dedv = robust::ratio(rb * robust::ratio(e_pert, uscale) - um, dv);
+    e_pert = eos_e_rT_with_P(e, rh, Tpert, l, P_T);
+    p_pert = robust::ratio(PressureFromPreferredE(e, l, rh, e_pert, P_T), uscale);
+    dpdT = robust::ratio((p_pert - pr), dT);
+    dedT = robust::ratio(rb * robust::ratio(e_pert, uscale) - um, dT);
+  }
+  SGM_HD static Real PressureFromPreferredE(const EosDev &e, Lam &l, Real r, Real s, Real P_T) {
+    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, l);
+    return P_T;
+  }
   // :1042-1105
   SGM_HD void Jacobian() {
     Real dpdv[NM], dedv[NM], dpdT[NM];
@@ -569,19 +592,56 @@ struct PteRhoT : PteBase<NM, NM> {
     const Real Tpert = Tnorm * (Tequil + dT);
 #pragma unroll
     for (int m = 0; m < NM; m++) {
-      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
-      const Real vf_pert = vfrac[m] + dv;
-      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
-      Real P_T;
-      Real e_pert = eos_e_rT_with_P(E(m), rho_pert, Tcur, lam[m], P_T);
-      Real p_pert = robust::ratio(B::PressureFromPreferredP(m, rho_pert, e_pert, P_T), uscale);
-      dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
-      dedv[m] = robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dv);
-      e_pert = eos_e_rT_with_P(E(m), rho[m], Tpert, lam[m], P_T);
-      p_pert = robust::ratio(B::PressureFromPreferredP(m, rho[m], e_pert, P_T), uscale);
-      dpdT[m] = robust::ratio((p_pert - press[m]), dT);
-      dedT_sum += robust::ratio(rhobar[m] * robust::ratio(e_pert, uscale) - u[m], dT);
+      Real dedT;
+      MaterialDerivs(E(m), lam[m], vfrac[m], rhobar[m], rho[m], press[m], u[m], Tcur, Tpert, dT,
+                     dpdv[m], dedv[m], dpdT[m], dedT);
+      dedT_sum += dedT;
     }
+    AssembleJacobian(dpdv, dedv, dpdT, dedT_sum);
+  }
+  // Jacobian() by a group of G >= NM lanes holding identical copies of the
+  // cell (lanes base .. base+G-1, this lane g): lane g evaluates material g's
+  // derivatives, then every lane gathers them, the lambda caches included, in
+  // material order. The values, the order of the dedT sum and the caches are
+  // those of Jacobian().
+  template <int G>
+  __device__ void JacobianGroup(int g, int base) {
+    static_assert(G >= NM, "one lane per material");
+    const Real Tcur = Tnorm * Tequil;
+    const Real dT = Tequil * P.derivative_eps;
+    const Real Tpert = Tnorm * (Tequil + dT);
+    const int m = g < NM ? g : 0;
+    int ei = B::eidx[0];
+    Real lr = lam[0].lRho, lt = lam[0].lT;
+#pragma unroll
+    for (int k = 1; k < NM; ++k) {
+      ei = (m == k) ? (int)B::eidx[k] : ei;
+      lr = sel(m == k, lam[k].lRho, lr);
+      lt = sel(m == k, lam[k].lT, lt);
+    }
+    Lam l;
+    l.present = lam[0].present;
+    l.lRho = lr;
+    l.lT = lt;
+    Real my_dpdv, my_dedv, my_dpdT, my_dedT;
+    MaterialDerivs(registry_entry(ei), l, pick<NM>(vfrac, m), pick<NM>(rhobar, m),
+                   pick<NM>(rho, m), pick<NM>(press, m), pick<NM>(u, m), Tcur, Tpert, dT,
+                   my_dpdv, my_dedv, my_dpdT, my_dedT);
+    Real dpdv[NM], dedv[NM], dpdT[NM];
+    Real dedT_sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      dpdv[k] = __shfl(my_dpdv, base + k, 64);
+      dedv[k] = __shfl(my_dedv, base + k, 64);
+      dpdT[k] = __shfl(my_dpdT, base + k, 64);
+      dedT_sum += __shfl(my_dedT, base + k, 64);
+      lam[k].lRho = __shfl(l.lRho, base + k, 64);
+      lam[k].lT = __shfl(l.lT, base + k, 64);
+    }
+    AssembleJacobian(dpdv, dedv, dpdT, dedT_sum);
+  }
+  SGM_HD void AssembleJacobian(const Real (&dpdv)[NM], const Real (&dedv)[NM],
+                               const Real (&dpdT)[NM], Real dedT_sum) {
     const Real dedv_s = pick<NM>(dedv, ms), dpdT_s = pick<NM>(dpdT, ms),
                dpdv_s = pick<NM>(dpdv, ms);
     jac[0] = dedT_sum;
@@ -1334,6 +1394,21 @@ struct PureTrialSteps : std::true_type {};
 template <int NM>
 struct PureTrialSteps<PtePT<NM>> : std::false_type {};
 
+// The Jacobian of a group's cell: RhoT spreads its per-material derivative
+// evaluations over the group's lanes; the other formulations evaluate it on
+// every lane.
+template <int G, class S>
+__device__ __forceinline__ void group_jacobian(S &s, int, int) {
+  s.Jacobian();
+}
+template <int G, int NM>
+__device__ __forceinline__ void group_jacobian(PteRhoT<NM> &s, int g, int base) {
+  if constexpr (G >= NM)
+    s.template JacobianGroup<G>(g, base);
+  else
+    s.Jacobian();
+}
+
 // One pass of the PTESolver loop body (pte_iterate) by a group of G lanes
 // (lanes base .. base+G-1, this lane g) that hold identical copies of one
 // cell. Everything up to the line search runs on every lane; the line
@@ -1351,7 +1426,7 @@ __device__ __forceinline__ bool pte_iterate_group(S &s, DriverState &d, CellStat
   st.niter = d.niter;
   s.CheckPTE(d.converged, d.close_enough);
   if (d.converged) return true;
-  s.Jacobian();
+  group_jacobian<G>(s, g, base);
   if (!s.Solve()) {
     d.converged = false;
     s.err |= ERRB_LINSOLVE;

@@ -331,17 +331,20 @@ __attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
                                    NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2))) pte_tail_kernel(PteArgs a,
                                                                       MixParamsDev P,
                                                                       ParkArgs pk,
-                                                                      int level) {
+                                                                      int level,
+                                                                      int resident) {
   constexpr int G = TailGroup(METHOD, NM);
   constexpr int per_slot = 64 / G;  // records per wave slot
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
   const int lane = (int)threadIdx.x;
   const bool two_levels = G > 1 && pk.after2 > 0 && pk.l2_count && P.line_search_max_iter <= G;
+  // level 2: groups while its cells fit the GPU's resident waves at once;
+  // past that the level is throughput-bound and keeps one lane per cell
   bool grp = false;
   if (level == 2) {
     if (!two_levels) return;
-    grp = (long)(*pk.l2_count) * G <= 64L * (long)gridDim.x;
+    grp = (long)(*pk.l2_count) * G <= 64L * (long)resident;
   }
   const int lpc = grp ? G : 1;  // lanes per cell
   const int spb = grp ? 1 : G;  // wave slots per block step
@@ -448,14 +451,17 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   const int g = g1 < cap ? g1 : cap;
   const bool l2 = TailGroup(METHOD, nm) > 1 && pk.after2 > 0 && pk.l2_count &&
                   P.line_search_max_iter <= TailGroup(METHOD, nm);
+  // level 2: one block per wave slot, so that no block walks two slots that
+  // both hold long-running cells (its few cells would then run one after the
+  // other); empty slots exit at once
   for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
-    const int gl = level == 1 ? g : (pk.nwaves < cap ? pk.nwaves : cap);
+    const int gl = level == 1 ? g : pk.nwaves;
     switch (nm) {
-    case 2: pte_tail_kernel<2, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
-    case 3: pte_tail_kernel<3, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
-    case 4: pte_tail_kernel<4, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
-    case 5: pte_tail_kernel<5, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
-    case 6: pte_tail_kernel<6, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level); break;
+    case 2: pte_tail_kernel<2, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
+    case 3: pte_tail_kernel<3, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
+    case 4: pte_tail_kernel<4, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
+    case 5: pte_tail_kernel<5, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
+    case 6: pte_tail_kernel<6, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
     default: return hipErrorInvalidValue;
     }
     const hipError_t e = hipGetLastError();
