@@ -213,6 +213,19 @@ int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                    double *press, const double *vfrac_tot, const double *P, double *lambda,
                    const sgm_mix_params *params, sgm_pte_status status, sgm_stream_t stream);
 
+/* Replaces PTESolverRhoU(nmat, eos, vfrac_tot, sie_tot, rho, vfrac, sie, temp,
+ * press, lambda, scratch, Tnorm, params) + PTESolver
+ * (mixed_cell_models.hpp:1924-2183, 2188-2291): the volume fractions and the
+ * energy densities of every material are solved for, so that the materials
+ * share one pressure and one temperature (each from T(rho, e) of its EOS).
+ * Arguments as sgm_pte_rhoT; outputs overwrite rho, vfrac, sie, temp (per
+ * material), press. Plain per-call kernels only (no batched plan). */
+int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+                 double *rho, double *vfrac, double *sie, double *temp, double *press,
+                 const double *vfrac_tot, const double *sie_tot, const double *Tnorm,
+                 double *lambda, const sgm_mix_params *params, sgm_pte_status status,
+                 sgm_stream_t stream);
+
 /* ---- closure driver ----------------------------------------------------- */
 /* Replaces get_sg_eos(nmat, ncell, cell_dim, input_int, eos_offsets, eos,
  * offsets, press, pmax, vol, spvol, sie, temp, bmod, dpde, cv, frac_mass,

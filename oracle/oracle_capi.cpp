@@ -290,10 +290,11 @@ void sgo_build_gruneisen_table(void *grv, double lRhoMin, double lRhoMax, int nR
 
 // ---- PTE closure over a CSR cell list ------------------------------------
 // method 0 = PTESolverRhoT, 1 = PTESolverPT, 2 = PTESolverFixedT,
-// 3 = PTESolverFixedP. offsets[ncell+1]; mat_eos[nnz] indexes eos_handles;
-// per-(cell,mat) arrays are in/out; lambda is nnz*3 or null; Tnorm may be null
-// (0 = no guess; methods 0 and 1 only). cell_in[c] is the cell's specific
-// internal energy (methods 0, 1), its temperature (2) or its pressure (3).
+// 3 = PTESolverFixedP, 4 = PTESolverRhoU. offsets[ncell+1]; mat_eos[nnz]
+// indexes eos_handles; per-(cell,mat) arrays are in/out; lambda is nnz*3 or
+// null; Tnorm may be null (0 = no guess; methods 0, 1 and 4 only). cell_in[c]
+// is the cell's specific internal energy (methods 0, 1, 4), its temperature
+// (2) or its pressure (3).
 // Returns number of unconverged cells.
 long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eos,
                    void **eos_handles, int neos, double *rho, double *vfrac, double *sie,
@@ -307,7 +308,8 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
   long nfail = 0;
 #pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : nfail)
   {
-    std::vector<Real> scratch(PTESolverFixedPRequiredScratch(8) + PTESolverRhoTRequiredScratch(8));
+    std::vector<Real> scratch(PTESolverFixedPRequiredScratch(8) + PTESolverRhoTRequiredScratch(8) +
+                              PTESolverRhoURequiredScratch(8));
     EOS eos_cell[8];
 #pragma omp for schedule(dynamic, 256)
     for (long c = 0; c < ncell; ++c) {
@@ -324,6 +326,10 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
       } else if (method == 1) {
         PTEPT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                 temp + off, press + off, lam, scratch.data(), tn, params);
+        st = PTESolver(s);
+      } else if (method == 4) {
+        PTERhoU s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+                  temp + off, press + off, lam, scratch.data(), tn, params);
         st = PTESolver(s);
       } else if (method == 2) {
         PTEFixedT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,

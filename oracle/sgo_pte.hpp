@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only. CPU restatement of the singularity-eos
 // pressure-temperature-equilibrium closure: MixParams, SolverStatus,
 // PTESolverBase, PTESolverRhoT, PTESolverPT, PTESolverFixedT,
-// PTESolverFixedP, the PTESolver Newton driver,
+// PTESolverFixedP, PTESolverRhoU, the PTESolver Newton driver,
 // ApproxTemperatureFromRhoMatU and MixUtils. Follows
 // singularity-eos/closure/mixed_cell_models.hpp line by line in algorithm;
 // the per-cell scratch layout mirrors the reference's AssignIncrement carving.
@@ -806,6 +806,229 @@ struct PTEFixedP : PTEBase {
     Residual();
     return ResidualNorm();
   }
+};
+
+// ---------------------------------------------------------------------------
+// PTESolverRhoU — mixed_cell_models.hpp:1924-2183 (with the base's
+// GetIdealPTE / TryIdealPTE, :672-763). Unknowns: the volume fractions and
+// the scaled energy densities u_m of every material (neq = 2 nmat); residual:
+// volume and energy closure, pressure and temperature differences of
+// neighbouring materials. Temperatures come from T(rho, e) of each EOS.
+// ---------------------------------------------------------------------------
+inline int PTESolverRhoURequiredScratch(int nmat) {
+  const int neq = 2 * nmat;
+  return neq * neq + 4 * neq + 8 * nmat;
+}
+
+struct PTERhoU : PTEBase {
+  Real *dpdv, *dtdv, *dpde, *dtde, *vtemp, *utemp;
+  PTERhoU(int nmat_, const EOS *eos_, Real vfrac_tot, Real sie_tot, Real *rho_, Real *vfrac_,
+          Real *sie_, Real *temp_, Real *press_, Real *lambda_, Real *scratch, Real Tnorm_,
+          const MixParams &p)
+      : PTEBase(nmat_, 2 * nmat_, eos_, vfrac_tot, sie_tot, rho_, vfrac_, sie_, temp_, press_,
+                lambda_, scratch, Tnorm_, p) {
+    dpdv = inc(scratch, nmat);
+    dtdv = inc(scratch, nmat);
+    dpde = inc(scratch, nmat);
+    dtde = inc(scratch, nmat);
+    vtemp = inc(scratch, nmat);
+    utemp = inc(scratch, nmat);
+  }
+  // :672-687 (scaled ideal-gas estimate of the equilibrium P and T)
+  void GetIdealPTE(Real &Pideal, Real &Tideal) const {
+    Real rhoBsum = 0.0;
+    Real Asum = 0.0;
+    for (int m = 0; m < nmat; m++) {
+      Asum += vfrac[m] * robust::ratio(press[m], temp[m]);
+      rhoBsum += rho[m] * vfrac[m] * robust::ratio(sie[m], temp[m]);
+    }
+    Asum *= uscale / Tnorm;
+    rhoBsum /= Tnorm;
+    Tideal = uscale / rhoBsum / Tnorm;
+    Pideal = robust::ratio(Tnorm * Tideal * Asum, uscale * vfrac_total);
+  }
+  // :691-763: replaces the initial guess by the ideal-gas estimate when that
+  // lowers the residual (the reference keeps its copies in the Jacobian
+  // scratch; here they are arrays of their own)
+  void TryIdealPTE() {
+    Real Pideal, Tideal;
+    GetIdealPTE(Pideal, Tideal);
+    std::vector<Real> etemp(nmat), ptemp(nmat), vt(nmat), rtemp(nmat), res(neq);
+    for (int m = 0; m < nmat; ++m) {
+      etemp[m] = u[m];
+      ptemp[m] = press[m];
+      vt[m] = vfrac[m];
+      rtemp[m] = rho[m];
+    }
+    Real res_norm_old = 0.0;
+    for (int m = 0; m < neq; ++m) {
+      res[m] = residual[m];
+      res_norm_old += res[m] * res[m];
+    }
+    const Real alpha = robust::ratio(Pideal, Tideal);
+    for (int m = 0; m < nmat; ++m) {
+      vfrac[m] *= robust::ratio(press[m], (temp[m] * alpha));
+      if (Tnorm * Tideal < 0 ||
+          robust::ratio(rhobar[m], vfrac[m]) < eos[m].RhoPmin(Tnorm * Tideal)) {
+        for (int n = 0; n <= m; n++) vfrac[n] = vt[n];
+        return;
+      }
+    }
+    for (int m = 0; m < nmat; ++m) {
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      const Real sie_m = eos[m].e_rT(rho[m], Tnorm * Tideal, lam(m));
+      u[m] = rhobar[m] * robust::ratio(sie_m, uscale);
+      press[m] = robust::ratio(
+          GetPressureFromPreferred(eos[m], rho[m], Tnorm * Tideal, sie_m, lam(m), false), uscale);
+    }
+    Residual();
+    Real res_norm_new = 0.0;
+    for (int m = 0; m < neq; m++) res_norm_new += residual[m] * residual[m];
+    if (res_norm_new > res_norm_old) {
+      for (int m = 0; m < nmat; ++m) {
+        vfrac[m] = vt[m];
+        rho[m] = rtemp[m];
+        u[m] = etemp[m];
+        press[m] = ptemp[m];
+      }
+      for (int m = 0; m < neq; ++m) residual[m] = res[m];
+    } else {
+      for (int m = 0; m < nmat; ++m) {
+        temp[m] = Tideal;
+        sie[m] = uscale * robust::ratio(u[m], rhobar[m]);
+      }
+    }
+  }
+  // :2005-2011
+  Real Init() {
+    InitBase();
+    Residual();
+    TryIdealPTE();
+    return ResidualNorm();
+  }
+  // :2014-2026
+  void Residual() const {
+    const Real vsum = mix::sum_neumaier(vfrac, nmat);
+    const Real esum = mix::sum_neumaier(u, nmat);
+    residual[0] = vfrac_total - vsum;
+    residual[1] = utotal_scale - esum;
+    for (int m = 0; m < nmat - 1; ++m) residual[2 + m] = press[m + 1] - press[m];
+    for (int m = nmat + 1; m < neq; m++) residual[m] = temp[m - nmat] - temp[m - nmat - 1];
+  }
+  // :2029-2055
+  std::pair<bool, bool> CheckPTE() const {
+    Real mean_p = vfrac[0] * press[0];
+    Real mean_t = rhobar[0] * temp[0];
+    Real error_p = 0.0;
+    Real error_t = 0.0;
+    for (int m = 1; m < nmat; ++m) {
+      mean_p += vfrac[m] * press[m];
+      mean_t += rhobar[m] * temp[m];
+      error_p += residual[m + 1] * residual[m + 1];
+      error_t += residual[m + nmat] * residual[m + nmat];
+    }
+    mean_t = robust::ratio(mean_t, rho_total);
+    error_p = std::sqrt(error_p);
+    error_t = std::sqrt(error_t);
+    const bool converged_p = (error_p < params_.pte_rel_tolerance_p * std::abs(mean_p) ||
+                              error_p < params_.pte_abs_tolerance_p);
+    const bool converged_t = (error_t < params_.pte_rel_tolerance_t * mean_t ||
+                              error_t < params_.pte_abs_tolerance_t);
+    const bool acceptable_p =
+        (error_p < params_.pte_rel_tolerance_p_sufficient * std::abs(mean_p) ||
+         error_p < params_.pte_abs_tolerance_p_sufficient);
+    const bool acceptable_t = (error_t < params_.pte_rel_tolerance_t_sufficient * mean_t ||
+                               error_t < params_.pte_abs_tolerance_t_sufficient);
+    return {converged_p && converged_t, acceptable_p && acceptable_t};
+  }
+  // :2058-2121
+  void Jacobian() const {
+    for (int m = 0; m < nmat; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * params_.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      Real t_pert = robust::ratio(eos[m].T_re(rho_pert, sie[m], lam(m)), Tnorm);
+      Real p_pert = robust::ratio(
+          GetPressureFromPreferred(eos[m], rho_pert, Tnorm * t_pert, sie[m], lam(m), false),
+          uscale);
+      dpdv[m] = std::min(robust::ratio(p_pert - press[m], dv), -std::abs(params_.pte_max_dpdv));
+      dtdv[m] = robust::ratio(t_pert - temp[m], dv);
+      const Real de = std::abs(u[m]) * params_.derivative_eps;
+      const Real e_pert = robust::ratio(u[m] + de, rhobar[m]);
+      t_pert = robust::ratio(eos[m].T_re(rho[m], uscale * e_pert, lam(m)), Tnorm);
+      p_pert = robust::ratio(GetPressureFromPreferred(eos[m], rho[m], Tnorm * t_pert,
+                                                      uscale * e_pert, lam(m), false),
+                             uscale);
+      dpde[m] = robust::ratio(p_pert - press[m], de);
+      dtde[m] = robust::ratio(t_pert - temp[m], de);
+      if (std::abs(dtde[m]) < params_.min_dtde) dtde[m] = params_.derivative_eps;
+    }
+    const int n = nmat, N = 2 * nmat;
+    for (int i = 0; i < neq * neq; ++i) jacobian[i] = 0.0;
+    for (int m = 0; m < n; ++m) {
+      jacobian[m] = 1.0;
+      jacobian[2 * n + n + m] = 1.0;
+    }
+    jacobian[2 * N] = dpdv[0];
+    jacobian[n * N + n - 1] = -dpdv[n - 1];
+    jacobian[(n + 1) * N] = dtdv[0];
+    jacobian[(2 * n - 1) * N + n - 1] = -dtdv[n - 1];
+    jacobian[2 * N + n] = dpde[0];
+    jacobian[n * N + 2 * n - 1] = -dpde[n - 1];
+    jacobian[(n + 1) * N + n] = dtde[0];
+    jacobian[(2 * n - 1) * N + 2 * n - 1] = -dtde[n - 1];
+    for (int m = 1; m < n - 1; ++m) {
+      jacobian[(1 + m) * N + m] = -dpdv[m];
+      jacobian[(2 + m) * N + m] = dpdv[m];
+      jacobian[(n + m) * N + m] = -dtdv[m];
+      jacobian[(n + m + 1) * N + m] = dtdv[m];
+      jacobian[(1 + m) * N + n + m] = -dpde[m];
+      jacobian[(2 + m) * N + n + m] = dpde[m];
+      jacobian[(n + m) * N + n + m] = -dtde[m];
+      jacobian[(n + m + 1) * N + n + m] = dtde[m];
+    }
+  }
+  // :2124-2158
+  Real ScaleDx() const {
+    Real scale = 1.0;
+    for (int m = 0; m < nmat; ++m) {
+      if (scale * dx[m] < -0.1 * vfrac[m]) scale = -0.1 * robust::ratio(vfrac[m], dx[m]);
+      const Real dt = (dtdv[m] * dx[m] + dtde[m] * dx[m + nmat]);
+      if (scale * dt < -0.1 * temp[m]) scale = -0.1 * robust::ratio(temp[m], dt);
+      const Real tt = temp[m] + scale * dt;
+      const Real rho_min =
+          std::max(eos[m].RhoPmin(Tnorm * temp[m]), eos[m].RhoPmin(Tnorm * tt));
+      const Real alpha_max = robust::ratio(rhobar[m], rho_min);
+      if (alpha_max < vfrac[m]) continue;
+      if (scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+        scale = 0.5 * robust::ratio(alpha_max - vfrac[m], dx[m]);
+    }
+    for (int i = 0; i < neq; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :2162-2181
+  Real TestUpdate(const Real scale, bool cache_state = false) {
+    if (cache_state) {
+      for (int m = 0; m < nmat; ++m) {
+        vtemp[m] = vfrac[m];
+        utemp[m] = u[m];
+      }
+    }
+    for (int m = 0; m < nmat; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      u[m] = utemp[m] + scale * dx[nmat + m];
+      sie[m] = uscale * robust::ratio(u[m], rhobar[m]);
+      temp[m] = robust::ratio(eos[m].T_re(rho[m], sie[m], lam(m)), Tnorm);
+      press[m] = robust::ratio(
+          GetPressureFromPreferred(eos[m], rho[m], Tnorm * temp[m], sie[m], lam(m), false),
+          uscale);
+    }
+    Residual();
+    return ResidualNorm();
+  }
+  void Fixup() {}  // the base's hook (:346), not overridden by RhoU
+  void Finalize() { FinalizeBase(); }
 };
 
 // ---------------------------------------------------------------------------

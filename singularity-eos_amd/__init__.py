@@ -110,6 +110,8 @@ _SIGS = {
                           C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_PT": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                         C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_pte_rhoU": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                          C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_fixedT": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -282,9 +284,10 @@ class Context:
     def pte(self, method, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press, vfrac_tot,
             sie_tot, Tnorm=None, lam=None, params=None, conv=None, niter=None, resid=None,
             err=None, stream=0):
-        """method 0 PTESolverRhoT, 1 PTESolverPT (sie_tot: per-cell specific
-        internal energy), 2 PTESolverFixedT (sie_tot: per-cell temperature),
-        3 PTESolverFixedP (sie_tot: per-cell pressure); Tnorm: methods 0, 1."""
+        """method 0 PTESolverRhoT, 1 PTESolverPT, 4 PTESolverRhoU (sie_tot:
+        per-cell specific internal energy), 2 PTESolverFixedT (sie_tot: per-cell
+        temperature), 3 PTESolverFixedP (sie_tot: per-cell pressure); Tnorm:
+        methods 0, 1, 4."""
         st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
         pp = C.byref(params) if params is not None else None
         if method in (2, 3):
@@ -292,7 +295,7 @@ class Context:
             return f(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids), ptr(rho), ptr(vfrac),
                      ptr(sie), ptr(temp), ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(lam), pp,
                      st, stream)
-        f = lib().sgm_pte_rhoT if method == 0 else lib().sgm_pte_PT
+        f = {0: lib().sgm_pte_rhoT, 1: lib().sgm_pte_PT, 4: lib().sgm_pte_rhoU}[method]
         return f(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids), ptr(rho), ptr(vfrac), ptr(sie),
                  ptr(temp), ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm), ptr(lam), pp,
                  st, stream)

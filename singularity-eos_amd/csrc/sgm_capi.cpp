@@ -517,4 +517,13 @@ int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                     press, vfrac_tot, P, nullptr, lambda, params, status, stream);
 }
 
+int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+                 double *rho, double *vfrac, double *sie, double *temp, double *press,
+                 const double *vfrac_tot, const double *sie_tot, const double *Tnorm,
+                 double *lambda, const sgm_mix_params *params, sgm_pte_status status,
+                 sgm_stream_t stream) {
+  return pte_common(PTE_RHOU, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press,
+                    vfrac_tot, sie_tot, Tnorm, lambda, params, status, stream);
+}
+
 } // extern "C"

@@ -106,7 +106,15 @@ struct ParkArgs {
 };
 
 // Closure formulations, by the method ids of the C ABI.
-enum PteMethod { PTE_RHOT = 0, PTE_PT = 1, PTE_FIXED_T = 2, PTE_FIXED_P = 3, PTE_NMETHODS = 4 };
+// (RhoU runs on the plain and list kernels only; batched plans take 0..3)
+enum PteMethod {
+  PTE_RHOT = 0,
+  PTE_PT = 1,
+  PTE_FIXED_T = 2,
+  PTE_FIXED_P = 3,
+  PTE_RHOU = 4,
+  PTE_NMETHODS = 5
+};
 
 // Lanes per parked cell in the tail kernels: a group of 8 lanes runs each
 // cell's line-search trials side by side (pte_iterate_group); PT's trials
@@ -138,6 +146,9 @@ SGM_PTE_LAUNCHERS(pt)
 SGM_PTE_LAUNCHERS(fixedt)
 SGM_PTE_LAUNCHERS(fixedp)
 #undef SGM_PTE_LAUNCHERS
+hipError_t launch_pte_plain_rhou(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
+hipError_t launch_pte_list_rhou(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                                const long *range, hipStream_t s);
 
 // the plain per-call kernel: one lane per cell, 1..8 materials
 inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParamsDev &P,
@@ -147,6 +158,7 @@ inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParams
   case PTE_PT: return launch_pte_plain_pt(a, P, s);
   case PTE_FIXED_T: return launch_pte_plain_fixedt(a, P, s);
   case PTE_FIXED_P: return launch_pte_plain_fixedp(a, P, s);
+  case PTE_RHOU: return launch_pte_plain_rhou(a, P, s);
   default: return hipErrorInvalidValue;
   }
 }
@@ -159,6 +171,7 @@ inline hipError_t launch_pte_list(int method, const PteArgs &a, const MixParamsD
   case PTE_PT: return launch_pte_list_pt(a, P, cells, n, range, s);
   case PTE_FIXED_T: return launch_pte_list_fixedt(a, P, cells, n, range, s);
   case PTE_FIXED_P: return launch_pte_list_fixedp(a, P, cells, n, range, s);
+  case PTE_RHOU: return launch_pte_list_rhou(a, P, cells, n, range, s);
   default: return hipErrorInvalidValue;
   }
 }

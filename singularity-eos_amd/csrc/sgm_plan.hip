@@ -338,7 +338,7 @@ namespace sgm {
 int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
                 int method, int min_nm, sgm_plan **out, hipStream_t s) {
   if (!ctx || !out || ncell < 1 || nnz < 1 || nnz > 2147483647L || ncell > 2147483647L ||
-      !offsets || !mat_ids || method < 0 || method >= PTE_NMETHODS)
+      !offsets || !mat_ids || method < 0 || method >= PTE_RHOU)  // plans take methods 0..3
     return SGM_E_ARG;
   *out = nullptr;
   if (ctx->host_only) return SGM_E_NOGPU;

@@ -1198,6 +1198,281 @@ struct PteFixedP : PteBase<NM, NM + 1> {
 };
 
 // ---------------------------------------------------------------------------
+// PTESolverRhoU (:1924-2183, with the base's GetIdealPTE / TryIdealPTE,
+// :672-763). Unknowns: the volume fractions and scaled energy densities u_m of
+// every material (NEQ = 2 NM); residual: volume and energy closure, pressure
+// and temperature differences of neighbouring materials; temperatures from
+// T(rho, e) of each EOS. Scratch of the reference that lives only within one
+// iteration (dpdv, dpde) stays local to Jacobian; dtdv / dtde carry from
+// Jacobian to ScaleDx in the same iteration.
+// ---------------------------------------------------------------------------
+template <int NM>
+struct PteRhoU : PteBase<NM, 2 * NM> {
+  using B = PteBase<NM, 2 * NM>;
+  using B::dx;
+  using B::E;
+  using B::jac;
+  using B::lam;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::utotal_scale;
+  using B::vfrac;
+  using B::vfrac_total;
+  static constexpr bool kStoresTemp = true;
+  static constexpr bool kPerMaterialT = true;
+  static constexpr int NEQ = 2 * NM;
+  Real temp[NM];
+  Real vtemp[NM], utemp[NM];
+  Real dtdv[NM], dtde[NM];
+
+  SGM_HD void SetCellInputs(Real vfrac_tot, Real sie_tot, Real Tnorm_in) {
+    vfrac_total = vfrac_tot;
+    B::sie_total = sie_tot;
+    Tnorm = Tnorm_in;
+  }
+  // :2014-2026
+  SGM_HD void Residual() {
+    NeumaierSum vs, es;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      vs.add(vfrac[m]);
+      es.add(u[m]);
+    }
+    res[0] = vfrac_total - vs.result();
+    res[1] = utotal_scale - es.result();
+#pragma unroll
+    for (int m = 0; m < NM - 1; ++m) res[2 + m] = press[m + 1] - press[m];
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j) res[NM + 1 + j] = temp[j + 1] - temp[j];
+  }
+  // :672-687
+  SGM_HD void GetIdealPTE(Real &Pideal, Real &Tideal) const {
+    Real rhoBsum = 0.0;
+    Real Asum = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      Asum += vfrac[m] * robust::ratio(press[m], temp[m]);
+      rhoBsum += rho[m] * vfrac[m] * robust::ratio(sie[m], temp[m]);
+    }
+    Asum *= uscale / Tnorm;
+    rhoBsum /= Tnorm;
+    Tideal = uscale / rhoBsum / Tnorm;
+    Pideal = robust::ratio(Tnorm * Tideal * Asum, uscale * vfrac_total);
+  }
+  // :691-763
+  SGM_HD void TryIdealPTE() {
+    Real Pideal, Tideal;
+    GetIdealPTE(Pideal, Tideal);
+    Real etemp[NM], ptemp[NM], vt[NM], rtemp[NM], rs[NEQ];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      etemp[m] = u[m];
+      ptemp[m] = press[m];
+      vt[m] = vfrac[m];
+      rtemp[m] = rho[m];
+    }
+    Real res_norm_old = 0.0;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) {
+      rs[i] = res[i];
+      res_norm_old += rs[i] * rs[i];
+    }
+    const Real alpha = robust::ratio(Pideal, Tideal);
+    bool abort = false;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      if (!abort) {
+        vfrac[m] *= robust::ratio(press[m], (temp[m] * alpha));
+        if (Tnorm * Tideal < 0 ||
+            robust::ratio(rhobar[m], vfrac[m]) < eos_rho_pmin(E(m), Tnorm * Tideal)) {
+          // restore the volume fractions changed so far and keep the guess
+#pragma unroll
+          for (int n = 0; n < NM; n++)
+            if (n <= m) vfrac[n] = vt[n];
+          abort = true;
+        }
+      }
+    }
+    if (abort) return;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      Real P_T;
+      const Real sie_m = eos_e_rT_with_P(E(m), rho[m], Tnorm * Tideal, lam[m], P_T);
+      u[m] = rhobar[m] * robust::ratio(sie_m, uscale);
+      press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie_m, P_T), uscale);
+    }
+    Residual();
+    Real res_norm_new = 0.0;
+#pragma unroll
+    for (int i = 0; i < NEQ; i++) res_norm_new += res[i] * res[i];
+    if (res_norm_new > res_norm_old) {
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        vfrac[m] = vt[m];
+        rho[m] = rtemp[m];
+        u[m] = etemp[m];
+        press[m] = ptemp[m];
+      }
+#pragma unroll
+      for (int i = 0; i < NEQ; ++i) res[i] = rs[i];
+    } else {
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        temp[m] = Tideal;
+        sie[m] = uscale * robust::ratio(u[m], rhobar[m]);
+      }
+    }
+  }
+  // :2005-2011 (InitBase sets every scaled temperature to 1, :641)
+  SGM_HD Real Init(const Real (&Tin)[NM]) {
+    B::InitBase(Tin);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) temp[m] = 1.0;
+    Residual();
+    TryIdealPTE();
+    return B::ResidualNorm();
+  }
+  SGM_HD void Fixup() {}  // the base's hook (:346)
+  // :2029-2055 -> (converged, acceptable)
+  SGM_HD void CheckPTE(bool &conv, bool &ok) const {
+    Real mean_p = vfrac[0] * press[0];
+    Real mean_t = rhobar[0] * temp[0];
+    Real error_p = 0.0;
+    Real error_t = 0.0;
+#pragma unroll
+    for (int m = 1; m < NM; ++m) {
+      mean_p += vfrac[m] * press[m];
+      mean_t += rhobar[m] * temp[m];
+      error_p += res[m + 1] * res[m + 1];
+      error_t += res[m + NM] * res[m + NM];
+    }
+    mean_t = robust::ratio(mean_t, B::rho_total);
+    error_p = sqrt(error_p);
+    error_t = sqrt(error_t);
+    const bool cp = (error_p < P.pte_rel_tolerance_p * fabs(mean_p) ||
+                     error_p < P.pte_abs_tolerance_p);
+    const bool ct = (error_t < P.pte_rel_tolerance_t * mean_t ||
+                     error_t < P.pte_abs_tolerance_t);
+    const bool ap = (error_p < P.pte_rel_tolerance_p_sufficient * fabs(mean_p) ||
+                     error_p < P.pte_abs_tolerance_p_sufficient);
+    const bool at = (error_t < P.pte_rel_tolerance_t_sufficient * mean_t ||
+                     error_t < P.pte_abs_tolerance_t_sufficient);
+    conv = cp && ct;
+    ok = ap && at;
+  }
+  // :2058-2121
+  SGM_HD void Jacobian() {
+    Real dpdv[NM], dpde[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
+      const Real vf_pert = vfrac[m] + dv;
+      const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
+      Real t_pert = robust::ratio(eos_T_re(E(m), rho_pert, sie[m], lam[m]), Tnorm);
+      Real p_pert =
+          robust::ratio(B::PressureFromPreferred(m, rho_pert, Tnorm * t_pert, sie[m]), uscale);
+      dpdv[m] = smin(robust::ratio(p_pert - press[m], dv), -fabs(P.pte_max_dpdv));
+      dtdv[m] = robust::ratio(t_pert - temp[m], dv);
+      const Real de = fabs(u[m]) * P.derivative_eps;
+      const Real e_pert = robust::ratio(u[m] + de, rhobar[m]);
+      t_pert = robust::ratio(eos_T_re(E(m), rho[m], uscale * e_pert, lam[m]), Tnorm);
+      p_pert = robust::ratio(
+          B::PressureFromPreferred(m, rho[m], Tnorm * t_pert, uscale * e_pert), uscale);
+      dpde[m] = robust::ratio(p_pert - press[m], de);
+      dtde[m] = robust::ratio(t_pert - temp[m], de);
+      if (fabs(dtde[m]) < P.min_dtde) dtde[m] = P.derivative_eps;
+    }
+    constexpr int n = NM, N = NEQ;
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) jac[i] = 0.0;
+#pragma unroll
+    for (int m = 0; m < n; ++m) {
+      jac[m] = 1.0;
+      jac[2 * n + n + m] = 1.0;
+    }
+    jac[2 * N] = dpdv[0];
+    jac[n * N + n - 1] = -dpdv[n - 1];
+    jac[(n + 1) * N] = dtdv[0];
+    jac[(2 * n - 1) * N + n - 1] = -dtdv[n - 1];
+    jac[2 * N + n] = dpde[0];
+    jac[n * N + 2 * n - 1] = -dpde[n - 1];
+    jac[(n + 1) * N + n] = dtde[0];
+    jac[(2 * n - 1) * N + 2 * n - 1] = -dtde[n - 1];
+#pragma unroll
+    for (int m = 1; m < n - 1; ++m) {
+      jac[(1 + m) * N + m] = -dpdv[m];
+      jac[(2 + m) * N + m] = dpdv[m];
+      jac[(n + m) * N + m] = -dtdv[m];
+      jac[(n + m + 1) * N + m] = dtdv[m];
+      jac[(1 + m) * N + n + m] = -dpde[m];
+      jac[(2 + m) * N + n + m] = dpde[m];
+      jac[(n + m) * N + n + m] = -dtde[m];
+      jac[(n + m + 1) * N + n + m] = dtde[m];
+    }
+  }
+  // :2124-2158
+  SGM_HD Real ScaleDx() {
+    Real scale = 1.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      if (scale * dx[m] < -0.1 * vfrac[m]) scale = -0.1 * robust::ratio(vfrac[m], dx[m]);
+      const Real dt = (dtdv[m] * dx[m] + dtde[m] * dx[m + NM]);
+      if (scale * dt < -0.1 * temp[m]) scale = -0.1 * robust::ratio(temp[m], dt);
+      const Real tt = temp[m] + scale * dt;
+      const Real rho_min =
+          smax(eos_rho_pmin(E(m), Tnorm * temp[m]), eos_rho_pmin(E(m), Tnorm * tt));
+      const Real alpha_max = robust::ratio(rhobar[m], rho_min);
+      if (!(alpha_max < vfrac[m])) {
+        if (scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
+          scale = 0.5 * robust::ratio(alpha_max - vfrac[m], dx[m]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) dx[i] *= scale;
+    return scale;
+  }
+  // :2162-2181
+  SGM_HD Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        vtemp[m] = vfrac[m];
+        utemp[m] = u[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      vfrac[m] = vtemp[m] + scale * dx[m];
+      rho[m] = robust::ratio(rhobar[m], vfrac[m]);
+      u[m] = utemp[m] + scale * dx[NM + m];
+      sie[m] = uscale * robust::ratio(u[m], rhobar[m]);
+      temp[m] = robust::ratio(eos_T_re(E(m), rho[m], sie[m], lam[m]), Tnorm);
+      press[m] =
+          robust::ratio(B::PressureFromPreferred(m, rho[m], Tnorm * temp[m], sie[m]), uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  SGM_HD Real Tout() const { return temp[0] * Tnorm; }
+  SGM_HD Real TempOut(int m) const { return temp[m] * Tnorm; }
+  template <class F>
+  SGM_HD constexpr void Visit(F &f) {
+    B::VisitBase(f);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) f(temp[m]);
+  }
+  SGM_HD Real Pout(int m) const { return press[m] * uscale; }
+};
+
+// ---------------------------------------------------------------------------
 // PTESolver driver (:2188-2291)
 // ---------------------------------------------------------------------------
 // The driver is split into begin / one iteration / end so a lane can retire

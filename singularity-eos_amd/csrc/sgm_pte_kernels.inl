@@ -32,13 +32,23 @@ __device__ __forceinline__ void stage_registry(EosDev *lds, const EosDev *reg, i
   __syncthreads();
 }
 
-// closure formulation by method id: 0 RhoT, 1 PT, 2 FixedT, 3 FixedP
+// closure formulation by method id: 0 RhoT, 1 PT, 2 FixedT, 3 FixedP, 4 RhoU
 template <int NM, int METHOD>
 using SysOf = typename std::conditional<
     METHOD == 0, PteRhoT<NM>,
-    typename std::conditional<METHOD == 1, PtePT<NM>,
-                              typename std::conditional<METHOD == 2, PteFixedT<NM>,
-                                                        PteFixedP<NM>>::type>::type>::type;
+    typename std::conditional<
+        METHOD == 1, PtePT<NM>,
+        typename std::conditional<
+            METHOD == 2, PteFixedT<NM>,
+            typename std::conditional<METHOD == 3, PteFixedP<NM>, PteRhoU<NM>>::type>::type>::
+        type>::type;
+
+// formulations whose materials keep temperatures of their own (RhoU)
+template <class S, class = void>
+struct PerMaterialT : std::false_type {};
+template <class S>
+struct PerMaterialT<S, std::void_t<decltype(S::kPerMaterialT)>>
+    : std::integral_constant<bool, S::kPerMaterialT> {};
 
 template <int NM, class Sys>
 __device__ __forceinline__ void load_cell(Sys &s, Real (&Tin)[NM], const EosDev *lds,
@@ -75,7 +85,10 @@ __device__ __forceinline__ void store_cell(const Sys &s, const CellStatus &st, c
     a.rho[k] = s.rho[m];
     a.vfrac[k] = s.vfrac[m];
     a.sie[k] = s.sie[m];
-    if (Sys::kStoresTemp) a.temp[k] = Tout;
+    if constexpr (PerMaterialT<Sys>::value)
+      a.temp[k] = s.TempOut(m);
+    else if (Sys::kStoresTemp)
+      a.temp[k] = Tout;
     a.press[k] = s.Pout(m);
     if (a.lambda) {
       a.lambda[3L * k + 0] = s.lam[m].lRho;

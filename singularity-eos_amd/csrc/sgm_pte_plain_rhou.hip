@@ -1,0 +1,11 @@
+// Plain per-call PTE kernels, method PTESolverRhoU.
+#include "sgm_pte_kernels.inl"
+namespace sgm {
+hipError_t launch_pte_plain_rhou(const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
+  return kern::plain_launch<4>(a, P, s);
+}
+hipError_t launch_pte_list_rhou(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                                const long *range, hipStream_t s) {
+  return kern::list_launch<4>(a, P, cells, n, s, range);
+}
+} // namespace sgm
