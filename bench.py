@@ -351,23 +351,62 @@ def main():
             drv[k] = torch.zeros(n * nm_d, dtype=torch.float64, device=dev)
         offs = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
 
-        def driver_call():
+        def driver_call(mode, prm):
             for k, v in drv_in.items():
                 drv[k] = v.clone()
-            rc = sgm.get_sg_eos(ctx, 1, dc["eos_offsets"].tolist(), offs, drv, 1e-6,
-                                params=params, stream=stream)
+            rc = sgm.get_sg_eos(ctx, mode, dc["eos_offsets"].tolist(), offs, drv, 1e-6,
+                                params=prm, stream=stream)
             assert rc == 0, rc
 
-        driver_call()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        reps = 3
-        for _ in range(reps):
-            driver_call()
-        torch.cuda.synchronize()
-        ddt = (time.perf_counter() - t0) / reps
-        extras["get_sg_eos_rho_e_cells_per_s"] = n / ddt
-        log(f"[bench] get_sg_eos (rho-e) driver: {ddt * 1e3:.1f} ms per call of {n} cells")
+        # rho-e (PTESolverRhoT), rho-T (FixedT), rho-P (FixedP: its absolute
+        # pressure criterion cannot meet 1e-10, so the default tolerances),
+        # P-T (per-material inversions, no PTE)
+        drv_rates = {}
+        for mode, name, prm in ((1, "rho_e", params), (-3, "rho_T", params),
+                                (-2, "rho_P", sgm.mix_params()), (-1, "P_T", params)):
+            driver_call(mode, prm)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                driver_call(mode, prm)
+            torch.cuda.synchronize()
+            ddt = (time.perf_counter() - t0) / reps
+            drv_rates[name] = n / ddt
+            log(f"[bench] get_sg_eos ({name}) driver: {ddt * 1e3:.1f} ms per call of {n} cells")
+        extras["get_sg_eos_rho_e_cells_per_s"] = drv_rates["rho_e"]
+        extras["get_sg_eos_cells_per_s"] = drv_rates
+        # the other closure formulations, plain per-call path, 65,536 uneven
+        # cells: RhoU (cell energy), FixedT (the cell's first input
+        # temperature), FixedP (its volume-weighted input pressure, default
+        # tolerances)
+        u = wl.uneven(ctx, 65536, seed=0x5EED0000 + 7)
+        ucell = np.repeat(np.arange(u["ncell"]), np.diff(u["offsets"]))
+        cin = {4: u["sie_tot"], 2: u["temp"][u["offsets"][:-1]].copy(),
+               3: np.abs(np.bincount(ucell, weights=u["vfrac"] * u["press"])) + 1e5}
+        du = {k: torch.tensor(u[k], device=dev) for k in fields + ("vfrac_tot",)}
+        ou = torch.tensor(u["offsets"], device=dev)
+        mu = torch.tensor(u["mat_ids"], device=dev)
+        cu = torch.zeros(u["ncell"], dtype=torch.int32, device=dev)
+        form_rates = {}
+        for method, name in ((4, "RhoU"), (2, "FixedT"), (3, "FixedP")):
+            ci = torch.tensor(cin[method], device=dev)
+            prm = params if method != 3 else sgm.mix_params()
+            ts = []
+            for _ in range(3):
+                w = {k: du[k].clone() for k in fields}
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                rc = ctx.pte(method, u["ncell"], u["nnz"], ou, mu, w["rho"], w["vfrac"], w["sie"],
+                             w["temp"], w["press"], du["vfrac_tot"], ci, params=prm, conv=cu,
+                             stream=stream)
+                assert rc == 0, rc
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            form_rates[name] = u["ncell"] / min(ts[1:])
+            log(f"[bench] plain {name}, 65536 uneven cells: {form_rates[name]:.4g} cells/s, "
+                f"converged {cu.float().mean().item():.4f}")
+        extras["plain_formulations_65536_cells_per_s"] = form_rates
 
     if not args.no_serve:
         extras["serving"] = serve_bench(torch, ctx, a, params, args.serve_requests)
