@@ -234,6 +234,9 @@ int sgm_finalize(sgm_ctx *ctx) {
     return SGM_OK;
   }
   hipDeviceSynchronize();
+  if (ctx->drv_plan) sgm_plan_destroy(ctx->drv_plan);
+  if (ctx->drv_cells) hipFree(ctx->drv_cells);
+  if (ctx->drv_entries) hipFree(ctx->drv_entries);
   for (auto &t : ctx->tables)
     if (t)
       for (void *p : t->d_arrays) hipFree(p);

@@ -12,9 +12,12 @@
 //     volume, the mode's temperature / pressure / energy guesses, zeroed
 //     lambda caches);
 //  4. the batched PTE path (RhoT, FixedT or FixedP by mode) over the cells of
-//     two or more materials: a transient plan bins them by material count
-//     and signature and launches the register-resident persistent kernels
-//     (with straggler parking) on the caller's stream; then the epilogue:
+//     two or more materials: a plan bins them by material count and
+//     signature and launches the register-resident persistent kernels (with
+//     straggler parking) on the caller's stream; the context keeps the plan
+//     and reuses it while the next calls bring the same topology (an exact
+//     device comparison of the CSR offsets and material ids); then the
+//     epilogue:
 //     the single-material paths, the P-T mode's
 //     per-material inversions, and the reference's final_functor (cell
 //     averages, per-material bulk modulus, heat capacity, Gruneisen
@@ -29,6 +32,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace sgm;
@@ -266,10 +270,36 @@ __global__ void epilogue_kernel(DrvArgs a) {
   a.pmax[i] = a.press[i] > a.pmax[i] ? a.press[i] : a.pmax[i];
 }
 
-template <class T>
-hipError_t dalloc(T **p, size_t n) {
-  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
-}
+// Carves typed arrays (256-byte aligned) out of a grow-only device buffer:
+// take() records the layout, ok() grows the buffer when the layout does not
+// fit (after the stream's earlier work, which may still use it) and assigns
+// the pointers.
+struct Arena {
+  void *&buf;
+  size_t &cap;
+  size_t used = 0;
+  std::vector<std::pair<void **, size_t>> parts;
+  Arena(void *&b, size_t &c) : buf(b), cap(c) {}
+  template <class T>
+  void take(T *&p, size_t n) {
+    p = nullptr;
+    parts.push_back({(void **)&p, used});
+    used += (sizeof(T) * (n ? n : 1) + 255) / 256 * 256;
+  }
+  bool ok(hipStream_t s) {
+    if (used > cap) {
+      if (hipStreamSynchronize(s) != hipSuccess) return false;
+      if (buf) (void)hipFree(buf);
+      buf = nullptr;
+      cap = 0;
+      const size_t want = used + used / 4;
+      if (hipMalloc(&buf, want) != hipSuccess) return false;
+      cap = want;
+    }
+    for (auto &q : parts) *q.first = static_cast<unsigned char *>(buf) + q.second;
+    return true;
+  }
+};
 
 }  // namespace
 
@@ -320,57 +350,78 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
   a.d_eos = ctx->d_eos;
   a.nreg = ctx->nmat;
   hipStream_t s = (hipStream_t)stream;
-  std::vector<void *> tmp;
+  // the driver's buffers and plan live in the context (grow-only arenas,
+  // reused by the next call); one call at a time per context
+  std::unique_lock<std::mutex> lock(ctx->drv_mu);
   auto cleanup = [&](int rc) {
     hipStreamSynchronize(s);
-    for (void *q : tmp) (void)hipFree(q);
     return rc;
   };
-  auto track = [&](hipError_t e, void *q) {
-    if (e == hipSuccess) tmp.push_back(q);
-    return e == hipSuccess;
-  };
   const int nb = (ncell + 255) / 256;
-  if (!track(dalloc(&a.npte, ncell + 1), a.npte) ||
-      !track(dalloc(&a.mass_sum, ncell), a.mass_sum) ||
-      !track(dalloc(&a.eoff, ncell + 1), a.eoff) || !track(dalloc(&a.cell_in, ncell), a.cell_in) ||
-      !track(dalloc(&a.vtot, ncell), a.vtot))
-    return cleanup(SGM_E_HIP);
-  if (hipMemsetAsync(a.npte + ncell, 0, sizeof(int), s) != hipSuccess) return cleanup(SGM_E_HIP);
-  prologue_kernel<<<nb, 256, 0, s>>>(a);
-  if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
   size_t scan_bytes = 0;
-  void *scan_tmp = nullptr;
-  if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
-          hipSuccess ||
-      !track(hipMalloc(&scan_tmp, scan_bytes + 16), scan_tmp) ||
-      hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
-          hipSuccess)
-    return cleanup(SGM_E_HIP);
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int *)nullptr, (int *)nullptr,
+                                       ncell + 1, s) != hipSuccess)
+    return SGM_E_HIP;
+  {
+    Arena ar(ctx->drv_cells, ctx->drv_cells_bytes);
+    ar.take(a.npte, ncell + 1);
+    ar.take(a.mass_sum, ncell);
+    ar.take(a.eoff, ncell + 1);
+    ar.take(a.cell_in, ncell);
+    ar.take(a.vtot, ncell);
+    unsigned char *scan_tmp = nullptr;
+    ar.take(scan_tmp, scan_bytes + 16);
+    if (!ar.ok(s)) return cleanup(SGM_E_HIP);
+    if (hipMemsetAsync(a.npte + ncell, 0, sizeof(int), s) != hipSuccess)
+      return cleanup(SGM_E_HIP);
+    prologue_kernel<<<nb, 256, 0, s>>>(a);
+    if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
+    if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
+        hipSuccess)
+      return cleanup(SGM_E_HIP);
+  }
   int nnz = 0;
   if (hipMemcpyAsync(&nnz, a.eoff + ncell, sizeof(int), hipMemcpyDeviceToHost, s) !=
           hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return cleanup(SGM_E_HIP);
-  if (!track(dalloc(&a.emat, nnz), a.emat) || !track(dalloc(&a.eid, nnz), a.eid) ||
-      !track(dalloc(&a.rho, nnz), a.rho) || !track(dalloc(&a.vfrac, nnz), a.vfrac) ||
-      !track(dalloc(&a.esie, nnz), a.esie) || !track(dalloc(&a.etemp, nnz), a.etemp) ||
-      !track(dalloc(&a.epress, nnz), a.epress) || !track(dalloc(&a.lam, 3L * nnz), a.lam))
-    return cleanup(SGM_E_HIP);
+  {
+    Arena ar(ctx->drv_entries, ctx->drv_entries_bytes);
+    ar.take(a.emat, nnz);
+    ar.take(a.eid, nnz);
+    ar.take(a.rho, nnz);
+    ar.take(a.vfrac, nnz);
+    ar.take(a.esie, nnz);
+    ar.take(a.etemp, nnz);
+    ar.take(a.epress, nnz);
+    ar.take(a.lam, 3L * nnz);
+    if (!ar.ok(s)) return cleanup(SGM_E_HIP);
+  }
   fill_kernel<<<nb, 256, 0, s>>>(a);
   if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
+  // the context's plan for the last topology, reused while it repeats (a
+  // hydro code's mixed cells change little from one step to the next)
   if (input_int != IN_P_T && nnz > 0) {
     const int method = input_int == IN_RHO_T   ? PTE_FIXED_T
                        : input_int == IN_RHO_P ? PTE_FIXED_P
                                                : PTE_RHOT;
-    sgm_plan *plan = nullptr;
-    int rc = plan_create(ctx, ncell, nnz, a.eoff, a.eid, method, 2, &plan, s);
-    if (rc != SGM_OK) return cleanup(rc);
+    int rc = SGM_OK;
+    if (ctx->drv_plan) {
+      bool same = false;
+      rc = plan_match(ctx->drv_plan, ncell, nnz, a.eoff, a.eid, method, s, same);
+      if (rc != SGM_OK) return cleanup(rc);
+      if (!same) {
+        sgm_plan_destroy(ctx->drv_plan);
+        ctx->drv_plan = nullptr;
+      }
+    }
+    if (!ctx->drv_plan) {
+      rc = plan_create(ctx, ncell, nnz, a.eoff, a.eid, method, 2, &ctx->drv_plan, s);
+      if (rc != SGM_OK) return cleanup(rc);
+    }
     sgm_pte_status st{a.conv, nullptr, nullptr, nullptr};
-    rc = plan_launch(plan, a.rho, a.vfrac, a.esie, a.etemp, a.epress, a.vtot, a.cell_in, a.lam,
-                     params, st, s);
-    hipStreamSynchronize(s);  // the plan's buffers are freed below
-    sgm_plan_destroy(plan);
+    rc = plan_launch(ctx->drv_plan, a.rho, a.vfrac, a.esie, a.etemp, a.epress, a.vtot,
+                     a.cell_in, a.lam, params, st, s);
     if (rc != SGM_OK) return cleanup(rc);
   }
   epilogue_kernel<<<nb, 256, 0, s>>>(a);

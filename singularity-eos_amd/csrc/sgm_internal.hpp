@@ -47,6 +47,13 @@ struct sgm_ctx {
   sgm::EosDev *d_eos = nullptr;
   bool dirty = true;
   std::mutex mu;
+  // get_sg_eos keeps the batched plan of its last cell topology (CSR offsets
+  // and material ids of the participating entries) and reuses it while the
+  // topology repeats; drv_mu serializes the driver calls that use it
+  sgm_plan *drv_plan = nullptr;
+  void *drv_cells = nullptr, *drv_entries = nullptr;  // its grow-only work buffers
+  size_t drv_cells_bytes = 0, drv_entries_bytes = 0;
+  std::mutex drv_mu;
 };
 
 namespace sgm {
@@ -204,6 +211,9 @@ inline hipError_t launch_pte_tail(int method, int nm, const PteArgs &a, const Mi
   }
 }
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+// flag = 0 if a[0..n) == b[0..n) and c[0..m) == d[0..m), else 1
+hipError_t launch_compare_csr(const int *a, const int *b, long n, const uint8_t *c,
+                              const uint8_t *d, long m, int *flag, hipStream_t s);
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s,
                             const int *cost = nullptr, int threshold = 0);
@@ -218,6 +228,10 @@ MixParamsDev to_dev(const sgm_mix_params *p);
 // solves only its cells of two or more materials.
 int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
                 int method, int min_nm, sgm_plan **out, hipStream_t s);
+// match = the plan was made for this method and exactly this CSR topology
+// (a device comparison; synchronizes s)
+int plan_match(sgm_plan *p, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+               int method, hipStream_t s, bool &match);
 int plan_launch(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
                 double *press, const double *vfrac_tot, const double *sie_tot, double *lambda,
                 const sgm_mix_params *params, sgm_pte_status status, hipStream_t s);

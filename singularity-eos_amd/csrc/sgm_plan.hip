@@ -47,6 +47,7 @@ struct sgm_plan {
   int *d_run_count = nullptr, *d_wave_run = nullptr, *d_wave_first = nullptr;
   int nruns = 0;
   int *d_l2_counts = nullptr;    // per bin: cells saved for the tail's second level
+  int *d_flag = nullptr;         // plan_match's result
   int park_after = 20;           // Newton iterations before a cell is parked; 0 = never
   std::vector<Bin> bins;
   hipStream_t cap_stream = nullptr;
@@ -408,6 +409,21 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
   return SGM_OK;
 }
 
+int plan_match(sgm_plan *p, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
+               int method, hipStream_t s, bool &match) {
+  match = false;
+  if (!p || p->method != method || p->ncell != ncell || p->nnz != nnz) return SGM_OK;
+  if (!p->d_flag && hipMalloc(&p->d_flag, sizeof(int)) != hipSuccess) return SGM_E_HIP;
+  int differ = 1;
+  if (launch_compare_csr(p->d_offsets, offsets, ncell + 1, p->d_mat, mat_ids, nnz, p->d_flag,
+                         s) != hipSuccess ||
+      hipMemcpyAsync(&differ, p->d_flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return SGM_E_HIP;
+  match = differ == 0;
+  return SGM_OK;
+}
+
 int plan_launch(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
                 double *press, const double *vfrac_tot, const double *sie_tot, double *lambda,
                 const sgm_mix_params *params, sgm_pte_status status, hipStream_t s) {
@@ -730,6 +746,7 @@ int sgm_plan_destroy(sgm_plan *p) {
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   if (p->d_offsets) hipFree(p->d_offsets);
+  if (p->d_flag) hipFree(p->d_flag);
   if (p->d_mat) hipFree(p->d_mat);
   if (p->d_sorted) hipFree(p->d_sorted);
   if (p->d_counters) hipFree(p->d_counters);

@@ -41,6 +41,28 @@ hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+__global__ void compare_csr_kernel(const int *a, const int *b, long n, const uint8_t *c,
+                                   const uint8_t *d, long m, int *flag) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  bool diff = false;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n || i < m; i += stride)
+    diff = diff || (i < n && a[i] != b[i]) || (i < m && c[i] != d[i]);
+  if (diff) flag[0] = 1;  // every writer stores the same value
+}
+} // namespace
+
+hipError_t launch_compare_csr(const int *a, const int *b, long n, const uint8_t *c,
+                              const uint8_t *d, long m, int *flag, hipStream_t s) {
+  hipError_t e = launch_zero_ints(flag, 1, s);
+  if (e != hipSuccess) return e;
+  const long work = n > m ? n : m;
+  const long blocks = (work + 255) / 256;
+  const int g = (int)(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096);
+  compare_csr_kernel<<<g, 256, 0, s>>>(a, b, n, c, d, m, flag);
+  return hipGetLastError();
+}
+
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s, const int *cost,
                             int threshold) {

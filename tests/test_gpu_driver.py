@@ -86,3 +86,37 @@ def test_driver_rejects_bad_arguments(sgm, setup, torch_gpu):
     arrays["frac_cv"] = z
     assert sgm.get_sg_eos(ctx, 5, [1, 2], off, arrays, 0.0) == -1  # unknown mode
     assert sgm.get_sg_eos(ctx, 1, [1, 99], off, arrays, 0.0) == -2  # unknown material
+
+
+def test_driver_reuses_plan_only_for_the_same_topology(sgm, workloads, oracle, setup,
+                                                        torch_gpu):
+    """The context keeps the plan of the last call's topology: a repeated call
+    (same cells and materials, new states) and a call with another topology
+    after it both match the oracle bit for bit."""
+    torch = torch_gpu
+    ctx, pool, _ = setup
+    n = 2500
+    base, eos_offsets, offsets = inputs(workloads, ctx, n, 0x5EED0000 + 80)
+    params = sgm.mix_params(tight=True)
+    rng = np.random.default_rng(81)
+    for call in range(3):
+        arr = {k: v.copy() for k, v in base.items()}
+        offs = offsets
+        if call == 1:  # same topology, other energies and temperatures
+            arr["sie"] = arr["sie"] * rng.uniform(0.9, 1.1, n)
+            arr["temp"] = arr["temp"] * rng.uniform(0.9, 1.1, n)
+        if call == 2:  # another topology: a different subset of the cells
+            offs = offsets[: len(offsets) // 2]
+        ref = {k: v.copy() for k, v in arr.items()}
+        nfail, err, conv_o = oracle.get_sg_eos(pool, 1, eos_offsets, offs, ref, 1e-6,
+                                               params.as_oracle_array())
+        dev = {k: torch.tensor(v, device="cuda") for k, v in arr.items()}
+        conv = torch.full((len(offs),), -1, dtype=torch.int32, device="cuda")
+        assert sgm.get_sg_eos(ctx, 1, eos_offsets.tolist(), torch.tensor(offs, device="cuda"),
+                              dev, 1e-6, params=params, converged=conv) == 0
+        torch.cuda.synchronize()
+        for k in CELL_FIELDS + FRAC_FIELDS:
+            a, b = dev[k].cpu().numpy(), ref[k]
+            same = (a == b) | (np.isnan(a) & np.isnan(b))
+            assert same.all(), (call, k, int((~same).sum()))
+        assert np.array_equal(conv.cpu().numpy(), conv_o), call
