@@ -249,7 +249,11 @@ int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const u
  * reference places it inside the PTE solver scratch, where with three or
  * more materials it overlaps the solver's last arrays; only tabulated EOS
  * write the cache, so results without tables are the same); cells with more
- * than 8 participating materials are not solved (pte_converged 0). */
+ * than 8 participating materials are not solved (pte_converged 0). The
+ * context keeps the driver's work buffers and the batched plan of the last
+ * call's cell topology until sgm_finalize, and reuses the plan while calls
+ * bring the same topology; calls on one context are serialized, and each
+ * synchronizes `stream` before it returns. */
 int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
                    const int *eos_offsets, const int *offsets, double *press, double *pmax,
                    double *vol, double *spvol, double *sie, double *temp, double *bmod,
