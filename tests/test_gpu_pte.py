@@ -284,23 +284,28 @@ def test_reorder_by_cost_keeps_results(sgm, workloads, pool, torch_gpu):
     plan.close()
 
 
-@pytest.mark.parametrize("method", [0, 1, 3])
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
 def test_straggler_parking_keeps_results(sgm, workloads, pool, torch_gpu, method):
     """Parking cells between Newton iterations and resuming them in the tail
-    kernel changes only the schedule: with every cell parked after 2
-    iterations, the default threshold, and parking off, each output, flag and
-    iteration count is bit-identical to the plain per-call path."""
+    kernels changes only the schedule: with every cell parked after 1 or 2
+    iterations (so many that the tails' second level runs one lane per cell),
+    after 20 (the default: the second level runs line-search groups) and 40,
+    and with parking off, each output, flag and iteration count is
+    bit-identical to the plain per-call path, whose loop runs every pass (the
+    tails skip the passes of cells at a fixed point of the loop)."""
     torch = torch_gpu
     ctx, _, _ = pool
     a = workloads.uneven(ctx, 12000, seed=0x5EED0000 + 24)
     if method == 3:  # fixed pressure: the volume-weighted input pressure per cell
         cell = np.repeat(np.arange(a["ncell"]), np.diff(a["offsets"]))
         a["sie_tot"] = np.abs(np.bincount(cell, weights=a["vfrac"] * a["press"])) + 1e5
+    if method == 2:  # fixed temperature: each cell's first input temperature
+        a["sie_tot"] = a["temp"][a["offsets"][:-1]].copy()
     params = sgm.mix_params(tight=method != 3)
     g = run_plain(torch, ctx, a, method, params)
     d = dev(torch, a)
     plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], method)
-    for after in (2, 40, 0, 1):
+    for after in (2, 20, 40, 0, 1):
         plan.set_straggler_threshold(after)
         w = {k: d[k].clone() for k in FIELDS}
         for k in ("conv", "niter", "resid", "err"):
