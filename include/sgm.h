@@ -226,6 +226,19 @@ int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const u
                  double *lambda, const sgm_mix_params *params, sgm_pte_status status,
                  sgm_stream_t stream);
 
+/* Replaces ApproxTemperatureFromRhoMatU(nmat, eos, u_tot, rho, vfrac, Tguess,
+ * lambda) (mixed_cell_models.hpp:790-847), once per CSR cell: a rough
+ * temperature whose energy sum sum_m rho_m vfrac_m e_m(rho_m, T) brackets
+ * u_tot[c] (the cell's internal energy density rho*e, erg/cm^3), from
+ * bisection in log T over [1e-9, 1e9] K. Tguess: NULL (= 0) or [ncell];
+ * lambda: NULL or [nnz*3] (table caches, updated); T_out: [ncell] (NaN for
+ * a malformed cell). The usual start of PTESolverRhoT's Tnorm argument
+ * (test/test_pte_2phaseVinetSn.cpp:187-196). */
+int sgm_approx_temperature(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                           const uint8_t *mat_ids, const double *rho, const double *vfrac,
+                           const double *u_tot, const double *Tguess, double *lambda,
+                           double *T_out, sgm_stream_t stream);
+
 /* ---- closure driver ----------------------------------------------------- */
 /* Replaces get_sg_eos(nmat, ncell, cell_dim, input_int, eos_offsets, eos,
  * offsets, press, pmax, vol, spvol, sie, temp, bmod, dpde, cv, frac_mass,

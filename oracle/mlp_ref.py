@@ -51,16 +51,19 @@ def gelu_d(z):
     return 0.5 * (1 + t) + 0.5 * z * (1 - t * t) * k * (1 + 3 * c * z * z)
 
 
-def forward(P, X):
-    """X: [B, 64] (already bf16 values). -> Y [B, 16], cache"""
+def forward(P, X, quant=True):
+    """X: [B, 64] (already bf16 values). -> Y [B, 16], cache. quant=False
+    leaves out the bf16 rounding of weights and activations (the plain FP64
+    network, whose gradient finite differences can check)."""
     W, b = unpack(P)
-    Wq = [bf16(w) for w in W]
+    q = bf16 if quant else (lambda v: np.asarray(v, dtype=np.float64))
+    Wq = [q(w) for w in W]
     A = [np.asarray(X, dtype=np.float64)]
     Z = [None]
     for l in range(3):
         z = A[-1] @ Wq[l].T + b[l]
         Z.append(z)
-        A.append(bf16(gelu(z)))
+        A.append(q(gelu(z)))
     Y = A[-1] @ Wq[3].T + b[3]
     return Y, (A, Z, Wq)
 
@@ -87,9 +90,9 @@ def loss_grad(Y, T, wT=1.0):
     return L, dY
 
 
-def gradients(P, X, T, wT=1.0):
-    """Flat gradient in the device parameter layout."""
-    Y, (A, Z, Wq) = forward(P, X)
+def gradients(P, X, T, wT=1.0, quant=True):
+    """Flat gradient of the mean loss in the device parameter layout."""
+    Y, (A, Z, Wq) = forward(P, X, quant)
     L, dZ = loss_grad(Y, T, wT)
     g = np.zeros(NPARAM)
     offs = [(OW1, OB1), (OW2, OB2), (OW3, OB3), (OW4, OB4)]

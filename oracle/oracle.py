@@ -19,6 +19,9 @@ CXXFLAGS = ["-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenm
 
 
 FLOPS_LIB = os.path.join(BUILD, "liboracle_flops.so")
+# the same restatement with the host libm's exp / cbrt in Vinet in place of
+# the portable routines the device shares (loaded when SGO_ORACLE_LIBM=1)
+LIBM_LIB = os.path.join(BUILD, "liboracle_libm.so")
 
 
 def build(force=False):
@@ -28,7 +31,8 @@ def build(force=False):
     deps = [os.path.join(HERE, s) for s in SRCS + ["flops_count.cpp"]]
     deps += [os.path.join(HERE, h) for h in HDRS + ["sgo_real.hpp"]]
     newest = max(os.path.getmtime(d) for d in deps)
-    for out, src, extra in ((LIB, SRCS, []), (FLOPS_LIB, ["flops_count.cpp"], ["-DSGO_COUNT_FLOPS"])):
+    for out, src, extra in ((LIB, SRCS, []), (FLOPS_LIB, ["flops_count.cpp"], ["-DSGO_COUNT_FLOPS"]),
+                            (LIBM_LIB, SRCS, ["-DSGO_LIBM_EXP_CBRT"])):
         if not force and os.path.exists(out) and os.path.getmtime(out) >= newest:
             continue
         tmp = out + ".tmp"
@@ -109,9 +113,10 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        path = LIBM_LIB if os.environ.get("SGO_ORACLE_LIBM") == "1" else LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         for n, (r, a) in _SIGS.items():
             f = getattr(L, n)
             f.restype = r

@@ -356,11 +356,11 @@ struct Vinet {
   void F_DT(Real rho, Real T, Real *out) const {
     constexpr int pref0vp = -2, pref0vip = 2, maxind = 40;
     Real sumP = 0.0, sumB = 0.0, sumE = 0.0;
-    const Real x = pmath::cbrt(robust::ratio(rho0, rho));
+    const Real x = vmath::cbrt(robust::ratio(rho0, rho));
     const Real x2inv = robust::ratio(1.0, x * x);
     const Real onemx = 1.0 - x;
     const Real etatimes1mx = VIP[1] * onemx;
-    const Real expetatimes1mx = pmath::exp(etatimes1mx);
+    const Real expetatimes1mx = vmath::exp(etatimes1mx);
     for (int ind = maxind; ind >= 2; ind--) {
       sumP = d2tod40[pref0vp + ind] + onemx * sumP;
       sumB = d2tod40[pref0vp + ind] * ind + onemx * sumB;
@@ -430,7 +430,7 @@ struct Vinet {
     F_DT(rho, t, o);
     return o[7] * o[7] * rho;
   }
-  Real MinimumDensity() const { return pmath::cbrt(10 * robust::EPS()) * rho0; }
+  Real MinimumDensity() const { return vmath::cbrt(10 * robust::EPS()) * rho0; }
   // eos_vinet.hpp:373-395
   void fill(Real &rho, Real &temp, Real &sie, Real &press, Real &cv_, Real &bmod,
             unsigned long output, ErrFlag &err) const {

@@ -164,6 +164,21 @@ inline Real cbrt(Real x) { g_flops += 30; return Real(cbrt(val(x))); }
 } // namespace pmath
 #endif
 
+// The reference's Vinet calls std::exp and std::cbrt
+// (singularity-eos/eos/eos_vinet.hpp:235-239). The oracle, like the device
+// code, calls the portable pmath routines, so that host and device agree bit
+// for bit; built with -DSGO_LIBM_EXP_CBRT it calls the host libm instead, to
+// measure what that substitution changes.
+namespace vmath {
+#ifdef SGO_LIBM_EXP_CBRT
+inline double exp(double x) { return std::exp(x); }
+inline double cbrt(double x) { return std::cbrt(x); }
+#else
+using pmath::cbrt;
+using pmath::exp;
+#endif
+} // namespace vmath
+
 // spiner_common::to_log / from_log — singularity-eos/eos/eos_spiner_common.hpp:52-57
 inline Real to_log(const Real x, const Real offset) {
   return fastmath::log10(std::abs(std::max(x, -offset) + offset) + robust::EPS());

@@ -116,6 +116,7 @@ _SIGS = {
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_approx_temperature": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "sgm_get_sg_eos": (ci, [vp, ci, ci, ci, ci, vp, vp] + [vp] * 15 +
                        [cd, C.POINTER(MixParams), vp, vp]),
     "sgm_plan_create": (ci, [vp, cl, cl, vp, vp, ci, C.POINTER(vp), vp]),
@@ -299,6 +300,13 @@ class Context:
         return f(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids), ptr(rho), ptr(vfrac), ptr(sie),
                  ptr(temp), ptr(press), ptr(vfrac_tot), ptr(sie_tot), ptr(Tnorm), ptr(lam), pp,
                  st, stream)
+
+    def approx_temperature(self, ncell, nnz, offsets, mat_ids, rho, vfrac, u_tot, T_out,
+                           Tguess=None, lam=None, stream=0):
+        """ApproxTemperatureFromRhoMatU per CSR cell (u_tot: rho*e per cell)."""
+        return lib().sgm_approx_temperature(self.h, ncell, nnz, ptr(offsets), ptr(mat_ids),
+                                            ptr(rho), ptr(vfrac), ptr(u_tot), ptr(Tguess),
+                                            ptr(lam), ptr(T_out), stream)
 
 
 SG_FIELDS = ("press", "pmax", "vol", "spvol", "sie", "temp", "bmod", "dpde", "cv")

@@ -2,11 +2,14 @@
 // MixParams defaults and device info. Argument checks happen here, on the
 // host, before anything is launched.
 #include "sgm_internal.hpp"
+#include "sgm_approx.hpp"
 
 #include <cmath>
 #include <cstring>
 
 using namespace sgm;
+
+static_assert(sgm::ERRB_INPUT == SGM_ERRB_INPUT, "status bit of a malformed cell");
 
 namespace sgm {
 bool build_table_host(const sgm_table_desc &in, TableHost &T);
@@ -527,6 +530,24 @@ int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const u
                  sgm_stream_t stream) {
   return pte_common(PTE_RHOU, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press,
                     vfrac_tot, sie_tot, Tnorm, lambda, params, status, stream);
+}
+
+int sgm_approx_temperature(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
+                           const uint8_t *mat_ids, const double *rho, const double *vfrac,
+                           const double *u_tot, const double *Tguess, double *lambda,
+                           double *T_out, sgm_stream_t stream) {
+  if (!ctx || ncell < 0 || nnz < 0) return SGM_E_ARG;
+  if (ctx->host_only) return SGM_E_NOGPU;
+  if (ncell == 0) return SGM_OK;
+  if (!offsets || !mat_ids || !rho || !vfrac || !u_tot || !T_out) return SGM_E_ARG;
+  if (nnz > 2147483647L) return SGM_E_ARG;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
+  return launch_approx_temperature(ctx->d_eos, ctx->nmat, ncell, nnz, offsets, mat_ids, rho,
+                                   vfrac, u_tot, Tguess, lambda, T_out,
+                                   (hipStream_t)stream) == hipSuccess
+             ? SGM_OK
+             : SGM_E_HIP;
 }
 
 } // extern "C"

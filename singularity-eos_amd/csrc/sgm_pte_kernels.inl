@@ -17,7 +17,7 @@
 // VGPRs. Per cell: reads rho, vfrac, temp, press per material + sie_tot,
 // vfrac_tot (+Tnorm); writes rho, vfrac, sie, temp, press + status.
 #pragma once
-#include "sgm_internal.hpp"
+#include "sgm_kernels.hpp"
 
 #include <type_traits>
 
@@ -134,7 +134,7 @@ __device__ __forceinline__ void mark_bad(const PteArgs &a, long c) {
   if (a.conv) a.conv[c] = 0;
   if (a.niter) a.niter[c] = 0;
   if (a.resid) a.resid[c] = 0;
-  if (a.err) a.err[c] = SGM_ERRB_INPUT;
+  if (a.err) a.err[c] = ERRB_INPUT;
 }
 
 template <int METHOD>
