@@ -130,6 +130,23 @@ int sgm_set_Vinet(sgm_ctx *ctx, int matindex, double rho0, double T0, double B0,
                   double A0, double Cv0, double E0, double S0, const double *d2to40);
 /* replaces init_sg_SpinerDependsRhoT (singularity_eos.hpp:80) */
 int sgm_set_TableRhoT(sgm_ctx *ctx, int matindex, const sgm_table_desc *desc);
+/* Wraps material matindex (set beforehand) in the reference's modifiers, as
+ * the init_sg_* functions' enabled / vals arguments do
+ * (singularity_eos.hpp:30-93, singularity_eos_init_utils.hpp:25-41, 75-81):
+ * enabled[0] == 1 ScaledEOS by vals[0], enabled[1] == 1 ShiftedEOS by
+ * vals[1], composed as ScaledEOS<ShiftedEOS<model>> (eos/modifiers/
+ * scaled_eos.hpp, shifted_eos.hpp). HOST arrays enabled[4], vals[6]; the
+ * bilinear ramp (enabled[2], enabled[3]) is refused with SGM_E_KIND. Every
+ * vector call, FillEos, the closures and get_sg_eos then see the modified
+ * material. A later setter call on matindex clears the modifiers. */
+int sgm_set_modifiers(sgm_ctx *ctx, int matindex, const int *enabled, const double *vals);
+/* The context's material registry as device code reads it (no counterpart in
+ * the reference, where EOS objects are copied to the device by the caller):
+ * *d_registry receives the device pointer of the context's material records
+ * and *n their count, for kernels written against include/sgm_device.hpp
+ * (their stage_registry copies the records into LDS). Valid until the next
+ * material setter or sgm_finalize. */
+int sgm_device_registry(sgm_ctx *ctx, const void **d_registry, int *n);
 /* host-side query: kind of a material, or SGM_E_MAT */
 int sgm_get_kind(const sgm_ctx *ctx, int matindex);
 /* host-side query: derived table curve (0 PMax 1 sielTMax 2 dEdTMax 3 gm1Max
@@ -274,6 +291,17 @@ int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_in
                    double *frac_ie, double *frac_bmod, double *frac_dpde, double *frac_cv,
                    double mass_frac_cutoff, const sgm_mix_params *params,
                    int *pte_converged, sgm_stream_t stream);
+
+/* No reference counterpart: frees the work buffers and the batched plan that
+ * sgm_get_sg_eos keeps in the context between calls (for the 2^20-cell
+ * uneven workload about 0.5 GB of straggler records plus the arenas); the
+ * next call rebuilds them. This memory is outside any server's
+ * hbm_budget_bytes. sgm_finalize releases it too. */
+int sgm_release_driver_cache(sgm_ctx *ctx);
+/* No reference counterpart: out[4] (HOST) = {calls that built the driver's
+ * plan, calls that reused it (same topology), bytes of the work buffers,
+ * 1 if a plan is held}. */
+int sgm_driver_cache_info(sgm_ctx *ctx, long *out);
 
 /* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
 /* No reference counterpart: a plan bins the cells of a fixed CSR topology by

@@ -84,6 +84,7 @@ _SIGS = {
     "sgo_table_free": (None, [vp]),
     "sgo_eos_table": (vp, [vp]),
     "sgo_eos_free": (None, [vp]),
+    "sgo_eos_set_modifiers": (None, [vp, ci, ci, cd, cd]),
     "sgo_table_export": (cl, [vp, ci, vp]),
     "sgo_table_scalar": (cd, [vp, ci]),
     "sgo_table_interp": (None, [vp, ci, vp, vp, cl, vp]),
@@ -141,6 +142,11 @@ class Eos:
             lib().sgo_eos_free(self.h)
         except Exception:
             pass
+
+    def modify(self, scaled=False, shifted=False, scale=1.0, shift=0.0):
+        """ScaledEOS<ShiftedEOS<...>> (either may be off), in place; returns self."""
+        lib().sgo_eos_set_modifiers(self.h, int(scaled), int(shifted), float(scale), float(shift))
+        return self
 
     def vec(self, func, in1, in2=None, lam=None, nthreads=0):
         in1 = np.ascontiguousarray(in1, dtype=np.float64)

@@ -101,6 +101,17 @@ void *sgo_eos_table(void *t) {
   return e;
 }
 void sgo_eos_free(void *e) { delete static_cast<EOS *>(e); }
+// ScaledEOS<ShiftedEOS<T>> as init_sg_* builds it from enabled[0] (scaled),
+// enabled[1] (shifted), vals[0] (scale), vals[1] (shift)
+// (singularity_eos_init_utils.hpp:25-41, 75-76)
+void sgo_eos_set_modifiers(void *h, int scaled, int shifted, double scale, double shift) {
+  EOS *e = static_cast<EOS *>(h);
+  e->scaled = scaled != 0;
+  e->shifted = shifted != 0;
+  e->scale = scaled ? scale : 1.0;
+  e->inv_scale = scaled ? 1. / scale : 1.0;
+  e->shift = shifted ? shift : 0.0;
+}
 
 // Export a derived table: 0 PMax 1 sielTMax 2 dEdTMax 3 gm1Max 4 rho_at_pmin
 // 5 lTColdCrit 6 bModCold 7 dPdECold 8 dEdTCold 9 bMod (2-D). Returns length.

@@ -967,13 +967,19 @@ enum Kind { K_IDEAL = 0, K_STIFF = 1, K_GRUN = 2, K_VINET = 3, K_TABLE = 4 };
 
 struct EOS {
   int kind = K_IDEAL;
+  // ShiftedEOS / ScaledEOS (singularity-eos/eos/modifiers/shifted_eos.hpp,
+  // scaled_eos.hpp) in the one order the reference's C interface builds,
+  // ScaledEOS<ShiftedEOS<T>> (singularity_eos_init_utils.hpp:25-41): either
+  // may be off.
+  bool shifted = false, scaled = false;
+  Real shift = 0.0, scale = 1.0, inv_scale = 1.0;
   IdealGas ig{};
   StiffGas sg{};
   Gruneisen gr{};
   Vinet vn{};
   const SpinerRhoT *tb = nullptr;
 
-  unsigned long PreferredInput() const {
+  unsigned long PreferredInput_b() const {
     switch (kind) {
     case K_IDEAL: return IdealGas::preferred;
     case K_STIFF: return StiffGas::preferred;
@@ -982,7 +988,7 @@ struct EOS {
     default: return SpinerRhoT::preferred;
     }
   }
-  Real T_re(Real rho, Real sie, Real *lam) const {
+  Real T_re_b(Real rho, Real sie, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.T_re(rho, sie);
     case K_STIFF: return sg.T_re(rho, sie);
@@ -991,7 +997,7 @@ struct EOS {
     default: return tb->T_re(rho, sie, lam);
     }
   }
-  Real e_rT(Real rho, Real T, Real *lam) const {
+  Real e_rT_b(Real rho, Real T, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.e_rT(rho, T);
     case K_STIFF: return sg.e_rT(rho, T);
@@ -1000,7 +1006,7 @@ struct EOS {
     default: return tb->e_rT(rho, T, lam);
     }
   }
-  Real P_rT(Real rho, Real T, Real *lam) const {
+  Real P_rT_b(Real rho, Real T, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.P_rT(rho, T);
     case K_STIFF: return sg.P_rT(rho, T);
@@ -1009,7 +1015,7 @@ struct EOS {
     default: return tb->P_rT(rho, T, lam);
     }
   }
-  Real P_re(Real rho, Real sie, Real *lam) const {
+  Real P_re_b(Real rho, Real sie, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.P_re(rho, sie);
     case K_STIFF: return sg.P_re(rho, sie);
@@ -1018,7 +1024,7 @@ struct EOS {
     default: return tb->P_re(rho, sie, lam);
     }
   }
-  Real cv_rT(Real rho, Real T, Real *lam) const {
+  Real cv_rT_b(Real rho, Real T, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.Cv;
     case K_STIFF: return sg.Cv;
@@ -1027,7 +1033,7 @@ struct EOS {
     default: return tb->cv_rT(rho, T, lam);
     }
   }
-  Real cv_re(Real rho, Real sie, Real *lam) const {
+  Real cv_re_b(Real rho, Real sie, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.Cv;
     case K_STIFF: return sg.Cv;
@@ -1036,7 +1042,7 @@ struct EOS {
     default: return tb->cv_re(rho, sie, lam);
     }
   }
-  Real bmod_rT(Real rho, Real T, Real *lam) const {
+  Real bmod_rT_b(Real rho, Real T, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.bmod_rT(rho, T);
     case K_STIFF: return sg.bmod_rT(rho, T);
@@ -1045,7 +1051,7 @@ struct EOS {
     default: return tb->bmod_rT(rho, T, lam);
     }
   }
-  Real bmod_re(Real rho, Real sie, Real *lam) const {
+  Real bmod_re_b(Real rho, Real sie, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.bmod_re(rho, sie);
     case K_STIFF: return sg.bmod_re(rho, sie);
@@ -1058,7 +1064,7 @@ struct EOS {
   // (eos_ideal.hpp:140-144, eos_stiff.hpp:135-139), Gruneisen Gamma(min(rho,
   // rho_max)) (eos_gruneisen.hpp:134-138), Vinet A0 B0 / (Cv0 rho)
   // (eos_vinet.hpp:110-114), the table's dPdE / rho (see SpinerRhoT)
-  Real gruneisen_rT(Real rho, Real T, Real *lam) const {
+  Real gruneisen_rT_b(Real rho, Real T, Real *lam) const {
     switch (kind) {
     case K_IDEAL: return ig.gm1;
     case K_STIFF: return sg.gm1;
@@ -1070,7 +1076,7 @@ struct EOS {
   // the temperature of ValuesAtReferenceState: room temperature for IdealGas
   // and StiffGas (eos_ideal.hpp:224, eos_stiff.hpp:42), T0 for Gruneisen and
   // Vinet (eos_gruneisen.hpp:558, eos_vinet.hpp:408), TNormal for the table
-  Real T_reference() const {
+  Real T_reference_b() const {
     switch (kind) {
     case K_IDEAL:
     case K_STIFF: return 293;
@@ -1081,7 +1087,7 @@ struct EOS {
   }
   // EosBase::MinInternalEnergyFromDensity default = e(rho, 0) (eos_base.hpp:598-603);
   // IdealGas returns 0 (eos_ideal.hpp:95-99), tables the cold curve (:822-827)
-  Real min_e(Real rho) const {
+  Real min_e_b(Real rho) const {
     switch (kind) {
     case K_IDEAL: return 0.0;
     case K_STIFF: return sg.e_rT(rho, 0.);
@@ -1090,21 +1096,21 @@ struct EOS {
     default: return tb->sieCold_.interp(tb->lRho_(rho));
     }
   }
-  Real MinimumDensity() const {
+  Real MinimumDensity_b() const {
     if (kind == K_VINET) return vn.MinimumDensity();
     if (kind == K_TABLE) return tb->rho_(tb->lRhoMin_);
     return 0;
   }
-  Real MinimumTemperature() const {
+  Real MinimumTemperature_b() const {
     if (kind == K_TABLE) return tb->T_(tb->lTMin_);
     return 0;
   }
-  Real MaximumDensity() const {
+  Real MaximumDensity_b() const {
     if (kind == K_GRUN) return gr.rho_max;
     if (kind == K_TABLE) return tb->rhoMax_;
     return 1e100;
   }
-  Real MinimumPressure() const {
+  Real MinimumPressure_b() const {
     if (kind == K_GRUN) return gr.P_re(0, 0);
     if (kind == K_VINET) return -1e100;
     if (kind == K_TABLE) return tb->PMin_;
@@ -1112,17 +1118,17 @@ struct EOS {
   }
   // Gruneisen returns its maximum stable density here, exactly as the
   // reference does (eos_gruneisen.hpp:160-163).
-  Real MaximumPressureAtTemperature(Real T, ErrFlag &err) const {
+  Real MaximumPressureAtTemperature_b(Real T, ErrFlag &err) const {
     if (kind == K_GRUN) return gr.MaxStableDensityAtTemperature(T, err);
     return 1e100;
   }
-  Real RhoPmin(Real T) const {
+  Real RhoPmin_b(Real T) const {
     if (kind == K_TABLE) return tb->RhoPmin(T);
     return 0.0;
   }
   // EosBase::DensityEnergyFromPressureTemperature default (eos_base.hpp:630-671)
   // used by Vinet; the others override.
-  void rhoe_PT(Real press, Real temp, Real *lam, Real &rho, Real &sie, ErrFlag &err) const {
+  void rhoe_PT_b(Real press, Real temp, Real *lam, Real &rho, Real &sie, ErrFlag &err) const {
     switch (kind) {
     case K_IDEAL: ig.rhoe_PT(press, temp, rho, sie); return;
     case K_STIFF: sg.rhoe_PT(press, temp, rho, sie); return;
@@ -1131,9 +1137,9 @@ struct EOS {
     default: break;
     }
     constexpr Real DEFAULT_RHO_GUESS = 12;
-    const Real rhomin = std::max(RhoPmin(temp), MinimumDensity());
-    const Real rhomax = MaximumDensity();
-    auto PofRT = [&](Real r) { return P_rT(r, temp, lam); };
+    const Real rhomin = std::max(RhoPmin_b(temp), MinimumDensity_b());
+    const Real rhomax = MaximumDensity_b();
+    auto PofRT = [&](Real r) { return P_rT_b(r, temp, lam); };
     Real rhoguess = rho;
     if ((rhoguess <= rhomin) || (rhoguess >= rhomax)) {
       if ((rhomin < DEFAULT_RHO_GUESS) && (DEFAULT_RHO_GUESS < rhomax))
@@ -1142,9 +1148,9 @@ struct EOS {
         rhoguess = 0.5 * (rhomin + rhomax);
     }
     root::findRoot(PofRT, press, rhoguess, rhomin, rhomax, robust::EPS(), robust::EPS(), rho);
-    sie = e_rT(rho, temp, lam);
+    sie = e_rT_b(rho, temp, lam);
   }
-  void fill(Real &rho, Real &temp, Real &sie, Real &press, Real &cv, Real &bmod,
+  void fill_b(Real &rho, Real &temp, Real &sie, Real &press, Real &cv, Real &bmod,
             unsigned long output, Real *lam, ErrFlag &err) const {
     switch (kind) {
     case K_IDEAL: ig.fill(rho, temp, sie, press, cv, bmod, output, err); return;
@@ -1152,6 +1158,83 @@ struct EOS {
     case K_GRUN: gr.fill(rho, temp, sie, press, cv, bmod, output, err); return;
     case K_VINET: vn.fill(rho, temp, sie, press, cv, bmod, output, err); return;
     default: tb->fill(rho, temp, sie, press, cv, bmod, output, lam, err); return;
+    }
+  }
+
+  // ---- the material as the variant sees it: base model under the modifiers
+  // ScaledEOS forwards (scale rho, sie / scale) to ShiftedEOS, which forwards
+  // (rho, sie - shift) to the model; energies come back + shift, then x scale.
+  Real rho_in(Real rho) const { return scaled ? scale * rho : rho; }
+  Real sie_in(Real sie) const {
+    Real s = scaled ? inv_scale * sie : sie;
+    return shifted ? s - shift : s;
+  }
+  Real sie_out(Real e) const {
+    if (shifted) e = e + shift;
+    return scaled ? scale * e : e;
+  }
+  unsigned long PreferredInput() const { return PreferredInput_b(); }
+  Real T_re(Real rho, Real sie, Real *lam) const { return T_re_b(rho_in(rho), sie_in(sie), lam); }
+  // scaled_eos.hpp:80-84: the energy of ScaledEOS takes rho unscaled
+  Real e_rT(Real rho, Real T, Real *lam) const { return sie_out(e_rT_b(rho, T, lam)); }
+  Real P_rT(Real rho, Real T, Real *lam) const { return P_rT_b(rho_in(rho), T, lam); }
+  Real P_re(Real rho, Real sie, Real *lam) const { return P_re_b(rho_in(rho), sie_in(sie), lam); }
+  Real cv_rT(Real rho, Real T, Real *lam) const { return cv_rT_b(rho_in(rho), T, lam); }
+  Real cv_re(Real rho, Real sie, Real *lam) const {
+    return cv_re_b(rho_in(rho), sie_in(sie), lam);
+  }
+  Real bmod_rT(Real rho, Real T, Real *lam) const { return bmod_rT_b(rho_in(rho), T, lam); }
+  Real bmod_re(Real rho, Real sie, Real *lam) const {
+    return bmod_re_b(rho_in(rho), sie_in(sie), lam);
+  }
+  Real gruneisen_rT(Real rho, Real T, Real *lam) const {
+    return gruneisen_rT_b(rho_in(rho), T, lam);
+  }
+  Real T_reference() const { return T_reference_b(); }
+  // scaled_eos.hpp:88-91 (rho scaled, energy not), shifted_eos.hpp:81-84 (+ shift)
+  Real min_e(Real rho) const {
+    const Real e = min_e_b(rho_in(rho));
+    return shifted ? e + shift : e;
+  }
+  Real MinimumDensity() const { return scaled ? inv_scale * MinimumDensity_b() : MinimumDensity_b(); }
+  Real MinimumTemperature() const { return MinimumTemperature_b(); }
+  Real MaximumDensity() const { return scaled ? inv_scale * MaximumDensity_b() : MaximumDensity_b(); }
+  Real MinimumPressure() const { return MinimumPressure_b(); }
+  Real MaximumPressureAtTemperature(Real T, ErrFlag &err) const {
+    return MaximumPressureAtTemperature_b(T, err);
+  }
+  Real RhoPmin(Real T) const { return scaled ? inv_scale * RhoPmin_b(T) : RhoPmin_b(T); }
+  // shifted_eos.hpp:399-403, scaled_eos.hpp:328-335
+  void rhoe_PT(Real press, Real temp, Real *lam, Real &rho, Real &sie, ErrFlag &err) const {
+    rhoe_PT_b(press, temp, lam, rho, sie, err);
+    if (shifted) sie = sie + shift;
+    if (scaled) {
+      rho = rho * inv_scale;
+      sie = sie * scale;
+    }
+  }
+  // FillEos of ScaledEOS over ShiftedEOS (scaled_eos.hpp:148-166,
+  // shifted_eos.hpp:142-158): by the model's preferred input, (rho, T) or
+  // (rho, sie); any other preferred input (IdealGas's) is an error
+  void fill(Real &rho, Real &temp, Real &sie, Real &press, Real &cv, Real &bmod,
+            unsigned long output, Real *lam, ErrFlag &err) const {
+    if (!shifted && !scaled) {
+      fill_b(rho, temp, sie, press, cv, bmod, output, lam, err);
+      return;
+    }
+    const unsigned long pref = PreferredInput_b();
+    if (pref == (thermalqs::density | thermalqs::temperature)) {
+      Real srho = rho_in(rho);
+      Real &r = scaled ? srho : rho;
+      fill_b(r, temp, sie, press, cv, bmod, output, lam, err);
+      sie = sie_out(sie);
+    } else if (pref == (thermalqs::density | thermalqs::specific_internal_energy)) {
+      Real srho = rho_in(rho);
+      Real &r = scaled ? srho : rho;
+      Real senergy = sie_in(sie);
+      fill_b(r, temp, senergy, press, cv, bmod, output, lam, err);
+    } else {
+      err.code |= ERR_FILLEOS;
     }
   }
 };

@@ -116,6 +116,10 @@ _SIGS = {
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
+    "sgm_set_modifiers": (ci, [vp, ci, vp, vp]),
+    "sgm_release_driver_cache": (ci, [vp]),
+    "sgm_driver_cache_info": (ci, [vp, vp]),
+    "sgm_device_registry": (ci, [vp, C.POINTER(vp), C.POINTER(ci)]),
     "sgm_approx_temperature": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "sgm_get_sg_eos": (ci, [vp, ci, ci, ci, ci, vp, vp] + [vp] * 15 +
                        [cd, C.POINTER(MixParams), vp, vp]),
@@ -248,6 +252,13 @@ class Context:
         arr = (C.c_double * 39)(*(d2to40 if d2to40 is not None else [0.0] * 39))
         return check(lib().sgm_set_Vinet(self.h, m, rho0, T0, B0, BP0, A0, Cv0, E0, S0,
                                          C.cast(arr, vp)), "set_Vinet")
+
+    def set_modifiers(self, m, scaled=False, shifted=False, scale=1.0, shift=0.0):
+        """ScaledEOS<ShiftedEOS<...>> around material m (init_sg_* enabled/vals)."""
+        en = (C.c_int * 4)(1 if scaled else 0, 1 if shifted else 0, 0, 0)
+        va = (C.c_double * 6)(scale, shift, 0, 0, 0, 0)
+        return check(lib().sgm_set_modifiers(self.h, m, C.cast(en, vp), C.cast(va, vp)),
+                     "set_modifiers")
 
     def set_table(self, m, desc):
         return check(lib().sgm_set_TableRhoT(self.h, m, C.byref(desc)), "set_TableRhoT")

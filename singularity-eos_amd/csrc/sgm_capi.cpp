@@ -237,9 +237,7 @@ int sgm_finalize(sgm_ctx *ctx) {
     return SGM_OK;
   }
   hipDeviceSynchronize();
-  if (ctx->drv_plan) sgm_plan_destroy(ctx->drv_plan);
-  if (ctx->drv_cells) hipFree(ctx->drv_cells);
-  if (ctx->drv_entries) hipFree(ctx->drv_entries);
+  sgm_release_driver_cache(ctx);
   for (auto &t : ctx->tables)
     if (t)
       for (void *p : t->d_arrays) hipFree(p);
@@ -248,6 +246,50 @@ int sgm_finalize(sgm_ctx *ctx) {
   if (ctx->d_eos) hipFree(ctx->d_eos);
   delete ctx;
   return SGM_OK;
+}
+
+int sgm_release_driver_cache(sgm_ctx *ctx) {
+  if (!ctx) return SGM_E_ARG;
+  std::lock_guard<std::mutex> g(ctx->drv_mu);
+  if (ctx->drv_plan) sgm_plan_destroy(ctx->drv_plan);
+  if (ctx->drv_cells) (void)hipFree(ctx->drv_cells);
+  if (ctx->drv_entries) (void)hipFree(ctx->drv_entries);
+  ctx->drv_plan = nullptr;
+  ctx->drv_cells = ctx->drv_entries = nullptr;
+  ctx->drv_cells_bytes = ctx->drv_entries_bytes = 0;
+  return SGM_OK;
+}
+
+int sgm_driver_cache_info(sgm_ctx *ctx, long *out) {
+  if (!ctx || !out) return SGM_E_ARG;
+  std::lock_guard<std::mutex> g(ctx->drv_mu);
+  out[0] = ctx->drv_plan_builds;
+  out[1] = ctx->drv_plan_reuses;
+  out[2] = (long)(ctx->drv_cells_bytes + ctx->drv_entries_bytes);
+  out[3] = ctx->drv_plan ? 1 : 0;
+  return SGM_OK;
+}
+
+int sgm_set_modifiers(sgm_ctx *ctx, int m, const int *enabled, const double *vals) {
+  const int r = check_mat(ctx, m);
+  if (r != SGM_OK) return r;
+  if (!enabled || !vals) return SGM_E_ARG;
+  if (enabled[2] == 1 || enabled[3] == 1) return SGM_E_KIND;  // ramps: not on this path
+  const bool scaled = enabled[0] == 1, shifted = enabled[1] == 1;
+  // the reference's checks (scaled_eos.hpp:55-62, shifted_eos.hpp:56-59)
+  if (scaled && (!(std::fabs(vals[0]) > 0) || std::isnan(vals[0]) ||
+                 !(std::fabs(1. / vals[0]) > 0)))
+    return SGM_E_ARG;
+  if (shifted && std::isnan(vals[1])) return SGM_E_ARG;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  for (EosDev *e : {&ctx->host_eos[m], &ctx->host_eval[m]}) {
+    e->mods = (shifted ? MOD_SHIFTED : 0) | (scaled ? MOD_SCALED : 0);
+    e->shift = shifted ? vals[1] : 0.0;
+    e->scale = scaled ? vals[0] : 1.0;
+    e->inv_scale = scaled ? 1. / vals[0] : 1.0;
+  }
+  ctx->dirty = true;
+  return sync_registry(ctx);
 }
 
 int sgm_get_kind(const sgm_ctx *ctx, int m) {
@@ -530,6 +572,16 @@ int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const u
                  sgm_stream_t stream) {
   return pte_common(PTE_RHOU, ctx, ncell, nnz, offsets, mat_ids, rho, vfrac, sie, temp, press,
                     vfrac_tot, sie_tot, Tnorm, lambda, params, status, stream);
+}
+
+int sgm_device_registry(sgm_ctx *ctx, const void **d_registry, int *n) {
+  if (!ctx || !d_registry || !n) return SGM_E_ARG;
+  if (ctx->host_only) return SGM_E_NOGPU;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
+  *d_registry = ctx->d_eos;
+  *n = ctx->nmat;
+  return SGM_OK;
 }
 
 int sgm_approx_temperature(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,

@@ -346,7 +346,10 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
   a.fc = frac_cv;
   a.cutoff = mass_frac_cutoff;
   a.conv = pte_converged;
-  if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (sync_registry(ctx) != SGM_OK) return SGM_E_HIP;
+  }
   a.d_eos = ctx->d_eos;
   a.nreg = ctx->nmat;
   hipStream_t s = (hipStream_t)stream;
@@ -413,11 +416,14 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
       if (!same) {
         sgm_plan_destroy(ctx->drv_plan);
         ctx->drv_plan = nullptr;
+      } else {
+        ++ctx->drv_plan_reuses;
       }
     }
     if (!ctx->drv_plan) {
       rc = plan_create(ctx, ncell, nnz, a.eoff, a.eid, method, 2, &ctx->drv_plan, s);
       if (rc != SGM_OK) return cleanup(rc);
+      ++ctx->drv_plan_builds;
     }
     sgm_pte_status st{a.conv, nullptr, nullptr, nullptr};
     rc = plan_launch(ctx->drv_plan, a.rho, a.vfrac, a.esie, a.etemp, a.epress, a.vtot,

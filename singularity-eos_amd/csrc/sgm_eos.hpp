@@ -49,7 +49,7 @@ struct TableDev {
   const Real *rho_at_pmin;
 };
 
-// 256-byte POD describing one material; the closure and vector kernels stage
+// 288-byte POD describing one material; the closure and vector kernels stage
 // the registry into LDS, so everything an evaluation needs besides table and
 // coefficient arrays is one LDS read away. p[] holds the model parameters:
 //  IdealGas  {Cv, gm1}
@@ -61,9 +61,17 @@ struct TableDev {
 // every coefficient they touch is exactly zero (the sums stay exactly 0
 // there, so skipping them leaves every bit of the result unchanged). A table
 // material carries its whole TableDev record (grid, scalars, array pointers).
+// Modifiers (singularity-eos/eos/modifiers/shifted_eos.hpp, scaled_eos.hpp)
+// in the one order the reference's C interface builds them,
+// ScaledEOS<ShiftedEOS<model>> (singularity_eos_init_utils.hpp:25-41):
+// MOD_SHIFTED and MOD_SCALED in `mods`, either may be off.
+enum EosMods : int { MOD_SHIFTED = 1, MOD_SCALED = 2 };
 struct alignas(16) EosDev {
   int kind;
   int nvinet_skip;
+  int mods;
+  int pad_;
+  Real shift, scale, inv_scale, pad2_;
   union {
     struct {
       Real p[12];
@@ -72,7 +80,7 @@ struct alignas(16) EosDev {
     TableDev tab;
   };
 };
-static_assert(sizeof(EosDev) == 256, "EosDev is 256 bytes");
+static_assert(sizeof(EosDev) == 288, "EosDev is 288 bytes");
 
 // Lambda cache for the table EOS: (lRho, lT) per point, used as root-find guesses
 struct Lam {
@@ -593,7 +601,7 @@ SGM_HD Real bmod_re(const Real *p, const Real *c, int ns, Real rho, Real sie) {
 // ---------------------------------------------------------------------------
 // Variant-style dispatch over EosDev (singularity-eos/eos/eos_variant.hpp)
 // ---------------------------------------------------------------------------
-SGM_HD unsigned long preferred_input(const EosDev &e) {
+SGM_HD unsigned long base_preferred_input(const EosDev &e) {
   switch (e.kind) {
   case KIND_IDEAL:
     return tq::density | tq::specific_internal_energy | tq::temperature | tq::pressure;
@@ -602,7 +610,7 @@ SGM_HD unsigned long preferred_input(const EosDev &e) {
   default: return tq::density | tq::temperature;
   }
 }
-SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+SGM_HD Real base_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::e_rT(e.m.p, rho, T);
   case KIND_STIFF: return stiff::e_rT(e.m.p, rho, T);
@@ -616,7 +624,7 @@ SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 // index and weight computation. The values equal separate eos_e_rT and
 // eos_P_rT calls bit for bit. For (rho, e)-preferred models P_T is 0 and the
 // caller evaluates P(rho, e).
-SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
+SGM_HD Real base_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
   switch (e.kind) {
   case KIND_VINET: {
     const vinet::EP o = vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
@@ -631,10 +639,10 @@ SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P
     P_T = table::P_from(t, rho, lRho, T, lT, w);
     return table::sie_from(t, lRho, T, lT, w);
   }
-  default: P_T = 0.0; return eos_e_rT(e, rho, T, lam);
+  default: P_T = 0.0; return base_e_rT(e, rho, T, lam);
   }
 }
-SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+SGM_HD Real base_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::P_rT(e.m.p, rho, T);
   case KIND_STIFF: return stiff::P_rT(e.m.p, rho, T);
@@ -645,12 +653,12 @@ SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 }
 // e(rho, T) and P(rho, T) for every model: the shared evaluation where there
 // is one (Vinet, table), separate closed forms otherwise
-SGM_HD Real eos_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
-  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return eos_e_rT_with_P(e, rho, T, lam, P);
-  P = eos_P_rT(e, rho, T, lam);
-  return eos_e_rT(e, rho, T, lam);
+SGM_HD Real base_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
+  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return base_e_rT_with_P(e, rho, T, lam, P);
+  P = base_P_rT(e, rho, T, lam);
+  return base_e_rT(e, rho, T, lam);
 }
-SGM_HD Real eos_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+SGM_HD Real base_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::P_re(e.m.p, rho, sie);
   case KIND_STIFF: return stiff::P_re(e.m.p, rho, sie);
@@ -659,7 +667,7 @@ SGM_HD Real eos_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   default: return table::P_re(e.tab, rho, sie, lam);
   }
 }
-SGM_HD Real eos_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+SGM_HD Real base_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::T_re(e.m.p, rho, sie);
   case KIND_STIFF: return stiff::T_re(e.m.p, rho, sie);
@@ -668,7 +676,7 @@ SGM_HD Real eos_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   default: return table::T_re(e.tab, rho, sie, lam);
   }
 }
-SGM_HD Real eos_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+SGM_HD Real base_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL:
   case KIND_STIFF: return e.m.p[0];
@@ -677,7 +685,7 @@ SGM_HD Real eos_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   default: return table::cv_rT(e.tab, rho, T, lam);
   }
 }
-SGM_HD Real eos_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+SGM_HD Real base_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL:
   case KIND_STIFF: return e.m.p[0];
@@ -686,7 +694,7 @@ SGM_HD Real eos_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   default: return table::cv_re(e.tab, rho, sie, lam);
   }
 }
-SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+SGM_HD Real base_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::bmod_rT(e.m.p, rho, T);
   case KIND_STIFF: return stiff::bmod_rT(e.m.p, rho, T);
@@ -699,7 +707,7 @@ SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 // (eos_ideal.hpp:140-144, eos_stiff.hpp:135-139), Gruneisen Gamma(min(rho,
 // rho_max)) (eos_gruneisen.hpp:134-138), Vinet A0 B0 / (Cv0 rho)
 // (eos_vinet.hpp:110-114), the table's dPdE / rho
-SGM_HD Real eos_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+SGM_HD Real base_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL:
   case KIND_STIFF: return e.m.p[1];
@@ -711,7 +719,7 @@ SGM_HD Real eos_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 // the temperature of ValuesAtReferenceState: room temperature for IdealGas
 // and StiffGas (eos_ideal.hpp:224, eos_stiff.hpp:42), T0 for Gruneisen and
 // Vinet (eos_gruneisen.hpp:558, eos_vinet.hpp:408), TNormal for the table
-SGM_HD Real eos_T_reference(const EosDev &e) {
+SGM_HD Real base_T_reference(const EosDev &e) {
   switch (e.kind) {
   case KIND_IDEAL:
   case KIND_STIFF: return 293;
@@ -720,7 +728,7 @@ SGM_HD Real eos_T_reference(const EosDev &e) {
   default: return table::T_normal(e.tab);
   }
 }
-SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+SGM_HD Real base_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
   switch (e.kind) {
   case KIND_IDEAL: return ideal::bmod_re(e.m.p, rho, sie);
   case KIND_STIFF: return stiff::bmod_re(e.m.p, rho, sie);
@@ -731,7 +739,7 @@ SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
 }
 // EosBase::MinInternalEnergyFromDensity = e(rho, 0) (eos_base.hpp:598-603);
 // IdealGas 0 (eos_ideal.hpp:95-99); Gruneisen clamps rho (:373-377); table cold curve.
-SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
+SGM_HD Real base_min_e(const EosDev &e, Real rho) {
   Lam none;
   switch (e.kind) {
   case KIND_IDEAL: return 0.0;
@@ -742,21 +750,21 @@ SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
   }
   (void)none;
 }
-SGM_HD Real eos_min_density(const EosDev &e) {
+SGM_HD Real base_min_density(const EosDev &e) {
   if (e.kind == KIND_VINET) return e.m.p[9];
   if (e.kind == KIND_TABLE) return table::rho_(e.tab, e.tab.lRhoMin);
   return 0;
 }
-SGM_HD Real eos_min_temperature(const EosDev &e) {
+SGM_HD Real base_min_temperature(const EosDev &e) {
   if (e.kind == KIND_TABLE) return table::T_(e.tab, e.tab.lTMin);
   return 0;
 }
-SGM_HD Real eos_max_density(const EosDev &e) {
+SGM_HD Real base_max_density(const EosDev &e) {
   if (e.kind == KIND_GRUN) return e.m.p[10];
   if (e.kind == KIND_TABLE) return e.tab.rhoMax;
   return 1e100;
 }
-SGM_HD Real eos_min_pressure(const EosDev &e) {
+SGM_HD Real base_min_pressure(const EosDev &e) {
   if (e.kind == KIND_GRUN) return grun::P_re(e.m.p, 0, 0);
   if (e.kind == KIND_VINET) return -1e100;
   if (e.kind == KIND_TABLE) return e.tab.PMin;
@@ -764,17 +772,17 @@ SGM_HD Real eos_min_pressure(const EosDev &e) {
 }
 // Gruneisen returns its maximum stable density here, as the reference does
 // (eos_gruneisen.hpp:160-163).
-SGM_HD Real eos_max_pressure_at_T(const EosDev &e, Real T, int &err) {
+SGM_HD Real base_max_pressure_at_T(const EosDev &e, Real T, int &err) {
   if (e.kind == KIND_GRUN) return grun::max_stable_density(e.m.p, T, err);
   return 1e100;
 }
-SGM_HD Real eos_rho_pmin(const EosDev &e, Real T) {
+SGM_HD Real base_rho_pmin(const EosDev &e, Real T) {
   if (e.kind == KIND_TABLE) return table::rho_pmin(e.tab, T);
   return 0.0;
 }
 // DensityEnergyFromPressureTemperature; rho is the in/out guess. The
 // EosBase default (eos_base.hpp:630-671, findRoot) serves Vinet.
-SGM_NOINL void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
+SGM_NOINL void base_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
                            int &err) {
   switch (e.kind) {
   case KIND_IDEAL: ideal::rhoe_PT(e.m.p, P, T, rho, sie); return;
@@ -800,7 +808,7 @@ SGM_NOINL void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie
 // FillEos for the (rho, sie) -> {T, P, cv, bmod} and (rho, T) -> {sie, P, cv, bmod}
 // input modes (eos_ideal.hpp:236-262, eos_stiff.hpp:203-229, eos_gruneisen.hpp:527-549,
 // eos_vinet.hpp:373-395, eos_spiner_rho_temp.hpp:1052-1101) plus (P, T) -> {rho, sie}.
-SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+SGM_HD void base_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
                      Real &bmod, unsigned long output, Lam &lam, int &err) {
   using namespace tq;
   const unsigned long input = ~output;
@@ -916,6 +924,168 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
     return;
   }
   }
+}
+
+// ---------------------------------------------------------------------------
+// The material as the variant sees it: the model under its modifiers.
+// ScaledEOS forwards (scale rho, sie / scale) to ShiftedEOS, which forwards
+// (rho, sie - shift) to the model; energies come back + shift, then x scale
+// (shifted_eos.hpp:64-158, 399-403; scaled_eos.hpp:66-166, 328-356).
+// Without modifiers each call is the model's own, inlined as before; a
+// modified material takes one out-of-line call (the mod_* functions), so
+// the closure kernels' inlined EOS code does not grow with the modifiers.
+// ---------------------------------------------------------------------------
+SGM_HD Real mod_rho(const EosDev &e, Real rho) { return (e.mods & MOD_SCALED) ? e.scale * rho : rho; }
+SGM_HD Real mod_sie(const EosDev &e, Real sie) {
+  const Real s = (e.mods & MOD_SCALED) ? e.inv_scale * sie : sie;
+  return (e.mods & MOD_SHIFTED) ? s - e.shift : s;
+}
+SGM_HD Real mod_energy_out(const EosDev &e, Real en) {
+  if (e.mods & MOD_SHIFTED) en = en + e.shift;
+  return (e.mods & MOD_SCALED) ? e.scale * en : en;
+}
+SGM_HD Real mod_rho_bound(const EosDev &e, Real r) {
+  return (e.mods & MOD_SCALED) ? e.inv_scale * r : r;
+}
+SGM_HD bool modified(const EosDev &e) { return __builtin_expect(e.mods != 0, 0); }
+
+// functions of (rho, T): rho scaled; `which`: 0 P, 1 cv, 2 bmod, 3 Gruneisen
+SGM_NOINL Real mod_f_rT(const EosDev &e, int which, Real rho, Real T, Lam &lam) {
+  const Real r = mod_rho(e, rho);
+  switch (which) {
+  case 0: return base_P_rT(e, r, T, lam);
+  case 1: return base_cv_rT(e, r, T, lam);
+  case 2: return base_bmod_rT(e, r, T, lam);
+  default: return base_gruneisen_rT(e, r, T, lam);
+  }
+}
+// functions of (rho, sie): rho scaled, sie unscaled and unshifted; `which`:
+// 0 P, 1 T, 2 cv, 3 bmod
+SGM_NOINL Real mod_f_re(const EosDev &e, int which, Real rho, Real sie, Lam &lam) {
+  const Real r = mod_rho(e, rho), s = mod_sie(e, sie);
+  switch (which) {
+  case 0: return base_P_re(e, r, s, lam);
+  case 1: return base_T_re(e, r, s, lam);
+  case 2: return base_cv_re(e, r, s, lam);
+  default: return base_bmod_re(e, r, s, lam);
+  }
+}
+// scaled_eos.hpp:80-84: ScaledEOS's energy takes rho unscaled. With P_T:
+// a scaled material evaluates its energy and its pressure at different
+// densities, so there the two are separate model calls.
+SGM_NOINL Real mod_e_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real *P_T) {
+  if (!P_T) return mod_energy_out(e, base_e_rT(e, rho, T, lam));
+  if (!(e.mods & MOD_SCALED)) return mod_energy_out(e, base_e_rT_with_P(e, rho, T, lam, *P_T));
+  const Real en = base_e_rT(e, rho, T, lam);
+  *P_T = (base_preferred_input(e) & tq::specific_internal_energy)
+             ? 0.0
+             : base_P_rT(e, mod_rho(e, rho), T, lam);
+  return mod_energy_out(e, en);
+}
+
+SGM_HD unsigned long preferred_input(const EosDev &e) { return base_preferred_input(e); }
+SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  if (modified(e)) return mod_e_rT(e, rho, T, lam, nullptr);
+  return base_e_rT(e, rho, T, lam);
+}
+SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  if (modified(e)) return mod_f_rT(e, 0, rho, T, lam);
+  return base_P_rT(e, rho, T, lam);
+}
+SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
+  if (modified(e)) return mod_e_rT(e, rho, T, lam, &P_T);
+  return base_e_rT_with_P(e, rho, T, lam, P_T);
+}
+SGM_HD Real eos_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
+  if (modified(e)) {
+    P = mod_f_rT(e, 0, rho, T, lam);
+    return mod_e_rT(e, rho, T, lam, nullptr);
+  }
+  return base_e_and_P_rT(e, rho, T, lam, P);
+}
+SGM_HD Real eos_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+  if (modified(e)) return mod_f_re(e, 0, rho, sie, lam);
+  return base_P_re(e, rho, sie, lam);
+}
+SGM_HD Real eos_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+  if (modified(e)) return mod_f_re(e, 1, rho, sie, lam);
+  return base_T_re(e, rho, sie, lam);
+}
+SGM_HD Real eos_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  if (modified(e)) return mod_f_rT(e, 1, rho, T, lam);
+  return base_cv_rT(e, rho, T, lam);
+}
+SGM_HD Real eos_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+  if (modified(e)) return mod_f_re(e, 2, rho, sie, lam);
+  return base_cv_re(e, rho, sie, lam);
+}
+SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  if (modified(e)) return mod_f_rT(e, 2, rho, T, lam);
+  return base_bmod_rT(e, rho, T, lam);
+}
+SGM_HD Real eos_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
+  if (modified(e)) return mod_f_rT(e, 3, rho, T, lam);
+  return base_gruneisen_rT(e, rho, T, lam);
+}
+SGM_HD Real eos_T_reference(const EosDev &e) { return base_T_reference(e); }
+SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
+  if (modified(e)) return mod_f_re(e, 3, rho, sie, lam);
+  return base_bmod_re(e, rho, sie, lam);
+}
+// scaled_eos.hpp:88-91 (rho scaled, energy not), shifted_eos.hpp:81-84 (+ shift)
+SGM_NOINL Real mod_min_e(const EosDev &e, Real rho) {
+  const Real en = base_min_e(e, mod_rho(e, rho));
+  return (e.mods & MOD_SHIFTED) ? en + e.shift : en;
+}
+SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
+  if (modified(e)) return mod_min_e(e, rho);
+  return base_min_e(e, rho);
+}
+SGM_HD Real eos_min_density(const EosDev &e) { return mod_rho_bound(e, base_min_density(e)); }
+SGM_HD Real eos_min_temperature(const EosDev &e) { return base_min_temperature(e); }
+SGM_HD Real eos_max_density(const EosDev &e) { return mod_rho_bound(e, base_max_density(e)); }
+SGM_HD Real eos_min_pressure(const EosDev &e) { return base_min_pressure(e); }
+SGM_HD Real eos_max_pressure_at_T(const EosDev &e, Real T, int &err) {
+  return base_max_pressure_at_T(e, T, err);
+}
+SGM_HD Real eos_rho_pmin(const EosDev &e, Real T) { return mod_rho_bound(e, base_rho_pmin(e, T)); }
+SGM_HD void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
+                        int &err) {
+  base_rhoe_PT(e, P, T, rho, sie, lam, err);
+  if (modified(e)) {
+    if (e.mods & MOD_SHIFTED) sie = sie + e.shift;
+    if (e.mods & MOD_SCALED) {
+      rho = rho * e.inv_scale;
+      sie = sie * e.scale;
+    }
+  }
+}
+// FillEos of ScaledEOS over ShiftedEOS (scaled_eos.hpp:148-166,
+// shifted_eos.hpp:142-158): by the model's preferred input, (rho, T) or
+// (rho, sie); a modified IdealGas (whose preferred input is neither) is the
+// reference's EOS_ERROR, here ERRB_FILLEOS
+SGM_NOINL void mod_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
+                        Real &cv, Real &bmod, unsigned long output, Lam &lam, int &err) {
+  const unsigned long pref = base_preferred_input(e);
+  Real srho = mod_rho(e, rho);
+  Real &r = (e.mods & MOD_SCALED) ? srho : rho;
+  if (pref == (tq::density | tq::temperature)) {
+    base_fill(e, r, temp, sie, press, cv, bmod, output, lam, err);
+    sie = mod_energy_out(e, sie);
+  } else if (pref == (tq::density | tq::specific_internal_energy)) {
+    Real senergy = mod_sie(e, sie);
+    base_fill(e, r, temp, senergy, press, cv, bmod, output, lam, err);
+  } else {
+    err |= ERRB_FILLEOS;
+  }
+}
+SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+                     Real &bmod, unsigned long output, Lam &lam, int &err) {
+  if (modified(e)) {
+    mod_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+    return;
+  }
+  base_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
 }
 
 } // namespace sgm

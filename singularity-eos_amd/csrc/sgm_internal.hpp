@@ -50,6 +50,7 @@ struct sgm_ctx {
   sgm_plan *drv_plan = nullptr;
   void *drv_cells = nullptr, *drv_entries = nullptr;  // its grow-only work buffers
   size_t drv_cells_bytes = 0, drv_entries_bytes = 0;
+  long drv_plan_builds = 0, drv_plan_reuses = 0;  // calls that built / reused the plan
   std::mutex drv_mu;
 };
 

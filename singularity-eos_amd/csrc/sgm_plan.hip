@@ -170,13 +170,20 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
   return e;
 }
 
+// frees the straggler records and run tables and clears every pointer, so a
+// later failed alloc_park or sgm_plan_destroy never frees them twice
 void free_park(sgm_plan *p) {
-  for (Bin &b : p->bins)
+  for (Bin &b : p->bins) {
     for (void *q : {(void *)b.park.rec, (void *)b.park.cell_of})
       if (q) (void)hipFree(q);
-  for (int *q : {p->d_run_of, p->d_run_base, p->d_run_cap, p->d_run_count, p->d_wave_run,
-                 p->d_wave_first, p->d_l2_counts})
-    if (q) (void)hipFree(q);
+    b.park.rec = nullptr;
+    b.park.cell_of = nullptr;
+  }
+  for (int **q : {&p->d_run_of, &p->d_run_base, &p->d_run_cap, &p->d_run_count, &p->d_wave_run,
+                  &p->d_wave_first, &p->d_l2_counts}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
 }
 
 // Resets the bin counters and the straggler claim counters (fill kernels, so
@@ -527,8 +534,13 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
   hipStream_t s = (hipStream_t)stream;
   const size_t nb = sizeof(double) * p->nnz;
   double *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  for (auto &q : w)
-    if (hipMalloc(&q, nb) != hipSuccess) return SGM_E_HIP;
+  for (auto &q : w) {
+    if (hipMalloc(&q, nb) != hipSuccess) {
+      for (double *x : w)
+        if (x) (void)hipFree(x);
+      return SGM_E_HIP;
+    }
+  }
   const MixParamsDev P = params_dev(params);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);

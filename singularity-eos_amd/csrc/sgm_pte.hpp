@@ -337,14 +337,14 @@ struct PteBase {
   // :599-620
   SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
     const EosDev &e = E(m);
-    if (e.kind == KIND_IDEAL) {
-      // IdealGas is the only model here whose preferred input includes
-      // pressure: MinimumPressure() = 0, MaximumPressureAtTemperature() = 1e100
-      // (eos_base.hpp:544-549), closed-form P-T inversion (eos_ideal.hpp:201-207)
-      const Real Pmin = 0.0, Pmax = 1e100;
+    if (preferred_input(e) & tq::pressure) {
+      // IdealGas (modified or not) is the only model here whose preferred
+      // input includes pressure: MinimumPressure() = 0,
+      // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549)
+      const Real Pmin = eos_min_pressure(e), Pmax = eos_max_pressure_at_T(e, T, err);
       const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
-      ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
-      cv = e.m.p[0];
+      eos_rhoe_PT(e, Pguess, T, r, s, lam[m], err);
+      cv = eos_cv_re(e, r, s, lam[m]);
     } else {
       s = eos_e_rT(e, r, T, lam[m]);
       cv = eos_cv_rT(e, r, T, lam[m]);
@@ -1482,7 +1482,9 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
 struct DriverState {
   Real err;
   int niter, small_step_iters, max_iter;
-  bool converged, close_enough;
+  // SolverStatus::max_line_niter and slow_convergence_detected
+  int max_line_niter;
+  bool converged, close_enough, slow;
 };
 
 template <class F>
@@ -1493,6 +1495,8 @@ SGM_HD constexpr void VisitDriver(DriverState &d, CellStatus &st, F &f) {
   f(d.max_iter);
   f(d.converged);
   f(d.close_enough);
+  f(d.max_line_niter);
+  f(d.slow);
   f(st.converged);
   f(st.niter);
   f(st.err);
@@ -1540,6 +1544,8 @@ SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &s
   d.max_iter = NM * s.P.pte_max_iter_per_mat;
   d.small_step_iters = 0;
   d.niter = 0;
+  d.max_line_niter = 0;
+  d.slow = false;
 }
 
 // one pass of `for (niter = 0; niter < max_iter; ++niter)`; true once the
@@ -1592,6 +1598,7 @@ SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
       err = e;
       if (e < err_old + p.line_search_alpha * scale * gradfdx) break;
       scale *= p.line_search_fac;
+      if (li > d.max_line_niter) d.max_line_niter = li;
       if (++li >= p.line_search_max_iter) break;
     }
   }
@@ -1600,6 +1607,7 @@ SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
   if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&
       d.close_enough) {
     d.converged = true;
+    d.slow = true;
     return true;
   }
   ++d.niter;
@@ -1755,6 +1763,9 @@ __device__ __forceinline__ bool pte_iterate_group(S &s, DriverState &d, CellStat
       }
       src = chosen;
       err = pick<G>(ks, chosen);
+      // the sequential loop records every trial index that failed
+      const int failed = found ? chosen - 1 : p.line_search_max_iter - 1;
+      if (failed > d.max_line_niter) d.max_line_niter = failed;
     }
   }
   group_take(s, d, st, base + src);
@@ -1763,6 +1774,7 @@ __device__ __forceinline__ bool pte_iterate_group(S &s, DriverState &d, CellStat
   if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&
       d.close_enough) {
     d.converged = true;
+    d.slow = true;
     return true;
   }
   ++d.niter;
