@@ -171,7 +171,9 @@ int sgm_vec_eval(sgm_ctx *ctx, int matindex, int func, const double *in1, const 
                  double *out, long n, int lambda_stride, double *lambda, sgm_stream_t stream);
 /* replaces Variant::FillEos(rhos, temps, energies, presses, cvs, bmods, num, output,
  * lambdas) (eos_variant.hpp:652-708). Arrays not read or written for `output`
- * may be NULL. errs: NULL or [n] SGM_ERRB_* bits. */
+ * may be NULL. errs: NULL or [n] SGM_ERRB_* bits. A point whose output set
+ * the model cannot fill (the reference's EOS_ERROR) gets SGM_ERRB_FILLEOS and
+ * keeps its array values. */
 int sgm_vec_fill_eos(sgm_ctx *ctx, int matindex, double *rho, double *temp, double *sie,
                      double *press, double *cv, double *bmod, long n, unsigned long output,
                      int lambda_stride, double *lambda, int *errs, sgm_stream_t stream);

@@ -122,12 +122,15 @@ __global__ void __launch_bounds__(256)
     }
     int err = 0;
     eos_fill(ek, r, t, s, p, c, b, output, lam, err);
-    if (rho && (output & tq::density)) rho[i] = r;
-    if (temp && (output & tq::temperature)) temp[i] = t;
-    if (sie && (output & tq::specific_internal_energy)) sie[i] = s;
-    if (press && (output & tq::pressure)) press[i] = p;
-    if (cv && (output & tq::specific_heat)) cv[i] = c;
-    if (bmod && (output & tq::bulk_modulus)) bmod[i] = b;
+    // an output set the model cannot fill (the reference's EOS_ERROR) writes
+    // nothing: the point's arrays keep their values
+    const bool filled = !(err & ERRB_FILLEOS);
+    if (filled && rho && (output & tq::density)) rho[i] = r;
+    if (filled && temp && (output & tq::temperature)) temp[i] = t;
+    if (filled && sie && (output & tq::specific_internal_energy)) sie[i] = s;
+    if (filled && press && (output & tq::pressure)) press[i] = p;
+    if (filled && cv && (output & tq::specific_heat)) cv[i] = c;
+    if (filled && bmod && (output & tq::bulk_modulus)) bmod[i] = b;
     if (lp) {
       lp[0] = lam.lRho;
       lp[1] = lam.lT;

@@ -29,7 +29,7 @@ def demo(sgm):
     f = L.sgm_demo_pte_per_cell
     vp = C.c_void_p
     f.restype = C.c_int
-    f.argtypes = [vp, C.c_int, C.c_long] + [vp] * 11 + [C.c_int, C.POINTER(sgm.MixParams)] + \
+    f.argtypes = [vp, C.c_int, C.c_long] + [vp] * 10 + [C.c_int, C.POINTER(sgm.MixParams)] + \
         [vp] * 5 + [vp]
     return f
 
