@@ -624,20 +624,40 @@ SGM_HD Real base_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 // index and weight computation. The values equal separate eos_e_rT and
 // eos_P_rT calls bit for bit. For (rho, e)-preferred models P_T is 0 and the
 // caller evaluates P(rho, e).
-SGM_HD Real base_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
+SGM_HD Real base_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T,
+                             Real rhoP) {
+  // rhoP: the density of the pressure (a scaled material's P sits at scale x
+  // rho, its energy at rho); where it differs from rho the model is
+  // evaluated a second time there, by the same code (a loop of one or two
+  // passes, kept rolled)
   switch (e.kind) {
   case KIND_VINET: {
-    const vinet::EP o = vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, T);
-    P_T = o.P;
-    return o.e;
+    Real en = 0.0, r = rho;
+#pragma clang loop unroll(disable)
+    for (int pass = 0; pass < 2; ++pass) {
+      const vinet::EP o = vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, r, T);
+      if (pass == 0) en = o.e;
+      P_T = o.P;
+      if (rhoP == rho) break;
+      r = rhoP;
+    }
+    return en;
   }
   case KIND_TABLE: {
     const TableDev &t = e.tab;
-    const Real lRho = table::lRho_(t, rho), lT = table::lT_(t, T);
-    lam.set(lRho, lT);
-    const int w = table::loc_rho_T(t, lRho, lT);
-    P_T = table::P_from(t, rho, lRho, T, lT, w);
-    return table::sie_from(t, lRho, T, lT, w);
+    const Real lT = table::lT_(t, T);
+    Real en = 0.0, r = rho;
+#pragma clang loop unroll(disable)
+    for (int pass = 0; pass < 2; ++pass) {
+      const Real lRho = table::lRho_(t, r);
+      lam.set(lRho, lT);
+      const int w = table::loc_rho_T(t, lRho, lT);
+      if (pass == 0) en = table::sie_from(t, lRho, T, lT, w);
+      P_T = table::P_from(t, r, lRho, T, lT, w);
+      if (rhoP == rho) break;
+      r = rhoP;
+    }
+    return en;
   }
   default: P_T = 0.0; return base_e_rT(e, rho, T, lam);
   }
@@ -654,7 +674,7 @@ SGM_HD Real base_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
 // e(rho, T) and P(rho, T) for every model: the shared evaluation where there
 // is one (Vinet, table), separate closed forms otherwise
 SGM_HD Real base_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
-  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return base_e_rT_with_P(e, rho, T, lam, P);
+  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return base_e_rT_with_P(e, rho, T, lam, P, rho);
   P = base_P_rT(e, rho, T, lam);
   return base_e_rT(e, rho, T, lam);
 }
@@ -782,7 +802,7 @@ SGM_HD Real base_rho_pmin(const EosDev &e, Real T) {
 }
 // DensityEnergyFromPressureTemperature; rho is the in/out guess. The
 // EosBase default (eos_base.hpp:630-671, findRoot) serves Vinet.
-SGM_NOINL void base_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
+SGM_HD void base_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
                            int &err) {
   switch (e.kind) {
   case KIND_IDEAL: ideal::rhoe_PT(e.m.p, P, T, rho, sie); return;
@@ -931,115 +951,87 @@ SGM_HD void base_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &p
 // ScaledEOS forwards (scale rho, sie / scale) to ShiftedEOS, which forwards
 // (rho, sie - shift) to the model; energies come back + shift, then x scale
 // (shifted_eos.hpp:64-158, 399-403; scaled_eos.hpp:66-166, 328-356).
-// Without modifiers each call is the model's own, inlined as before; a
-// modified material takes one out-of-line call (the mod_* functions), so
-// the closure kernels' inlined EOS code does not grow with the modifiers.
+// Each call is one model evaluation with its inputs and outputs mapped (an
+// exact pass-through without modifiers), inlined like the models; nothing is
+// called out of line, so no caller state has to leave registers. A scaled
+// material evaluates its energy at rho but its pressure at scale x rho, so
+// the fused e/P evaluation of a (rho, T) model runs its model code a second
+// time there.
 // ---------------------------------------------------------------------------
-SGM_HD Real mod_rho(const EosDev &e, Real rho) { return (e.mods & MOD_SCALED) ? e.scale * rho : rho; }
+// the modifier fields are read only on a modified material's path (a
+// branch, not a select), so an unmodified material's evaluation reads one
+// int more than the model's own
+SGM_HD int mods_of(const EosDev &e) {
+  int m = e.mods;
+  asm volatile("" : "+v"(m));
+  return m;
+}
+SGM_HD Real mod_rho(const EosDev &e, Real rho) {
+  if (__builtin_expect((mods_of(e) & MOD_SCALED) != 0, 0)) rho = e.scale * rho;
+  return rho;
+}
 SGM_HD Real mod_sie(const EosDev &e, Real sie) {
-  const Real s = (e.mods & MOD_SCALED) ? e.inv_scale * sie : sie;
-  return (e.mods & MOD_SHIFTED) ? s - e.shift : s;
+  const int m = mods_of(e);
+  if (__builtin_expect(m != 0, 0)) {
+    if (m & MOD_SCALED) sie = e.inv_scale * sie;
+    if (m & MOD_SHIFTED) sie = sie - e.shift;
+  }
+  return sie;
 }
 SGM_HD Real mod_energy_out(const EosDev &e, Real en) {
-  if (e.mods & MOD_SHIFTED) en = en + e.shift;
-  return (e.mods & MOD_SCALED) ? e.scale * en : en;
+  const int m = mods_of(e);
+  if (__builtin_expect(m != 0, 0)) {
+    if (m & MOD_SHIFTED) en = en + e.shift;
+    if (m & MOD_SCALED) en = e.scale * en;
+  }
+  return en;
 }
 SGM_HD Real mod_rho_bound(const EosDev &e, Real r) {
-  return (e.mods & MOD_SCALED) ? e.inv_scale * r : r;
+  if (__builtin_expect((mods_of(e) & MOD_SCALED) != 0, 0)) r = e.inv_scale * r;
+  return r;
 }
-SGM_HD bool modified(const EosDev &e) { return __builtin_expect(e.mods != 0, 0); }
-
-// functions of (rho, T): rho scaled; `which`: 0 P, 1 cv, 2 bmod, 3 Gruneisen
-SGM_NOINL Real mod_f_rT(const EosDev &e, int which, Real rho, Real T, Lam &lam) {
-  const Real r = mod_rho(e, rho);
-  switch (which) {
-  case 0: return base_P_rT(e, r, T, lam);
-  case 1: return base_cv_rT(e, r, T, lam);
-  case 2: return base_bmod_rT(e, r, T, lam);
-  default: return base_gruneisen_rT(e, r, T, lam);
-  }
-}
-// functions of (rho, sie): rho scaled, sie unscaled and unshifted; `which`:
-// 0 P, 1 T, 2 cv, 3 bmod
-SGM_NOINL Real mod_f_re(const EosDev &e, int which, Real rho, Real sie, Lam &lam) {
-  const Real r = mod_rho(e, rho), s = mod_sie(e, sie);
-  switch (which) {
-  case 0: return base_P_re(e, r, s, lam);
-  case 1: return base_T_re(e, r, s, lam);
-  case 2: return base_cv_re(e, r, s, lam);
-  default: return base_bmod_re(e, r, s, lam);
-  }
-}
-// scaled_eos.hpp:80-84: ScaledEOS's energy takes rho unscaled. With P_T:
-// a scaled material evaluates its energy and its pressure at different
-// densities, so there the two are separate model calls.
-SGM_NOINL Real mod_e_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real *P_T) {
-  if (!P_T) return mod_energy_out(e, base_e_rT(e, rho, T, lam));
-  if (!(e.mods & MOD_SCALED)) return mod_energy_out(e, base_e_rT_with_P(e, rho, T, lam, *P_T));
-  const Real en = base_e_rT(e, rho, T, lam);
-  *P_T = (base_preferred_input(e) & tq::specific_internal_energy)
-             ? 0.0
-             : base_P_rT(e, mod_rho(e, rho), T, lam);
-  return mod_energy_out(e, en);
-}
-
 SGM_HD unsigned long preferred_input(const EosDev &e) { return base_preferred_input(e); }
+// scaled_eos.hpp:80-84: ScaledEOS's energy takes rho unscaled
 SGM_HD Real eos_e_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
-  if (modified(e)) return mod_e_rT(e, rho, T, lam, nullptr);
-  return base_e_rT(e, rho, T, lam);
+  return mod_energy_out(e, base_e_rT(e, rho, T, lam));
 }
 SGM_HD Real eos_P_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
-  if (modified(e)) return mod_f_rT(e, 0, rho, T, lam);
-  return base_P_rT(e, rho, T, lam);
+  return base_P_rT(e, mod_rho(e, rho), T, lam);
 }
 SGM_HD Real eos_e_rT_with_P(const EosDev &e, Real rho, Real T, Lam &lam, Real &P_T) {
-  if (modified(e)) return mod_e_rT(e, rho, T, lam, &P_T);
-  return base_e_rT_with_P(e, rho, T, lam, P_T);
+  return mod_energy_out(e, base_e_rT_with_P(e, rho, T, lam, P_T, mod_rho(e, rho)));
 }
 SGM_HD Real eos_e_and_P_rT(const EosDev &e, Real rho, Real T, Lam &lam, Real &P) {
-  if (modified(e)) {
-    P = mod_f_rT(e, 0, rho, T, lam);
-    return mod_e_rT(e, rho, T, lam, nullptr);
-  }
-  return base_e_and_P_rT(e, rho, T, lam, P);
+  if (e.kind == KIND_VINET || e.kind == KIND_TABLE) return eos_e_rT_with_P(e, rho, T, lam, P);
+  P = eos_P_rT(e, rho, T, lam);
+  return eos_e_rT(e, rho, T, lam);
 }
 SGM_HD Real eos_P_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
-  if (modified(e)) return mod_f_re(e, 0, rho, sie, lam);
-  return base_P_re(e, rho, sie, lam);
+  return base_P_re(e, mod_rho(e, rho), mod_sie(e, sie), lam);
 }
 SGM_HD Real eos_T_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
-  if (modified(e)) return mod_f_re(e, 1, rho, sie, lam);
-  return base_T_re(e, rho, sie, lam);
+  return base_T_re(e, mod_rho(e, rho), mod_sie(e, sie), lam);
 }
 SGM_HD Real eos_cv_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
-  if (modified(e)) return mod_f_rT(e, 1, rho, T, lam);
-  return base_cv_rT(e, rho, T, lam);
+  return base_cv_rT(e, mod_rho(e, rho), T, lam);
 }
 SGM_HD Real eos_cv_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
-  if (modified(e)) return mod_f_re(e, 2, rho, sie, lam);
-  return base_cv_re(e, rho, sie, lam);
+  return base_cv_re(e, mod_rho(e, rho), mod_sie(e, sie), lam);
 }
 SGM_HD Real eos_bmod_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
-  if (modified(e)) return mod_f_rT(e, 2, rho, T, lam);
-  return base_bmod_rT(e, rho, T, lam);
+  return base_bmod_rT(e, mod_rho(e, rho), T, lam);
 }
 SGM_HD Real eos_gruneisen_rT(const EosDev &e, Real rho, Real T, Lam &lam) {
-  if (modified(e)) return mod_f_rT(e, 3, rho, T, lam);
-  return base_gruneisen_rT(e, rho, T, lam);
+  return base_gruneisen_rT(e, mod_rho(e, rho), T, lam);
 }
 SGM_HD Real eos_T_reference(const EosDev &e) { return base_T_reference(e); }
 SGM_HD Real eos_bmod_re(const EosDev &e, Real rho, Real sie, Lam &lam) {
-  if (modified(e)) return mod_f_re(e, 3, rho, sie, lam);
-  return base_bmod_re(e, rho, sie, lam);
+  return base_bmod_re(e, mod_rho(e, rho), mod_sie(e, sie), lam);
 }
 // scaled_eos.hpp:88-91 (rho scaled, energy not), shifted_eos.hpp:81-84 (+ shift)
-SGM_NOINL Real mod_min_e(const EosDev &e, Real rho) {
+SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
   const Real en = base_min_e(e, mod_rho(e, rho));
   return (e.mods & MOD_SHIFTED) ? en + e.shift : en;
-}
-SGM_HD Real eos_min_e(const EosDev &e, Real rho) {
-  if (modified(e)) return mod_min_e(e, rho);
-  return base_min_e(e, rho);
 }
 SGM_HD Real eos_min_density(const EosDev &e) { return mod_rho_bound(e, base_min_density(e)); }
 SGM_HD Real eos_min_temperature(const EosDev &e) { return base_min_temperature(e); }
@@ -1049,43 +1041,56 @@ SGM_HD Real eos_max_pressure_at_T(const EosDev &e, Real T, int &err) {
   return base_max_pressure_at_T(e, T, err);
 }
 SGM_HD Real eos_rho_pmin(const EosDev &e, Real T) { return mod_rho_bound(e, base_rho_pmin(e, T)); }
+// the model's P-T inversion out of line, its arguments and results by value
+// (a reference to the caller's state would keep that state out of registers)
+struct RhoSie {
+  Real rho, sie;
+  Lam lam;
+  int err;
+};
+SGM_NOINL RhoSie base_rhoe_PT_v(const EosDev *e, Real P, Real T, Real rho, Lam lam) {
+  RhoSie r;
+  r.rho = rho;
+  r.sie = 0.0;
+  r.err = 0;
+  base_rhoe_PT(*e, P, T, r.rho, r.sie, lam, r.err);
+  r.lam = lam;
+  return r;
+}
 SGM_HD void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, Lam &lam,
                         int &err) {
-  base_rhoe_PT(e, P, T, rho, sie, lam, err);
-  if (modified(e)) {
-    if (e.mods & MOD_SHIFTED) sie = sie + e.shift;
-    if (e.mods & MOD_SCALED) {
-      rho = rho * e.inv_scale;
-      sie = sie * e.scale;
-    }
+  const RhoSie v = base_rhoe_PT_v(&e, P, T, rho, lam);
+  rho = v.rho;
+  sie = v.sie;
+  lam = v.lam;
+  err |= v.err;
+  if (e.mods & MOD_SHIFTED) sie = sie + e.shift;
+  if (e.mods & MOD_SCALED) {
+    rho = rho * e.inv_scale;
+    sie = sie * e.scale;
   }
 }
 // FillEos of ScaledEOS over ShiftedEOS (scaled_eos.hpp:148-166,
 // shifted_eos.hpp:142-158): by the model's preferred input, (rho, T) or
 // (rho, sie); a modified IdealGas (whose preferred input is neither) is the
 // reference's EOS_ERROR, here ERRB_FILLEOS
-SGM_NOINL void mod_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
-                        Real &cv, Real &bmod, unsigned long output, Lam &lam, int &err) {
-  const unsigned long pref = base_preferred_input(e);
-  Real srho = mod_rho(e, rho);
-  Real &r = (e.mods & MOD_SCALED) ? srho : rho;
-  if (pref == (tq::density | tq::temperature)) {
-    base_fill(e, r, temp, sie, press, cv, bmod, output, lam, err);
-    sie = mod_energy_out(e, sie);
-  } else if (pref == (tq::density | tq::specific_internal_energy)) {
-    Real senergy = mod_sie(e, sie);
-    base_fill(e, r, temp, senergy, press, cv, bmod, output, lam, err);
-  } else {
-    err |= ERRB_FILLEOS;
-  }
-}
 SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
                      Real &bmod, unsigned long output, Lam &lam, int &err) {
-  if (modified(e)) {
-    mod_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+  if (!e.mods) {
+    base_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
     return;
   }
-  base_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+  const unsigned long pref = base_preferred_input(e);
+  const bool rhoT = pref == (tq::density | tq::temperature);
+  if (!rhoT && pref != (tq::density | tq::specific_internal_energy)) {
+    err |= ERRB_FILLEOS;
+    return;
+  }
+  Real srho = mod_rho(e, rho);
+  Real senergy = rhoT ? sie : mod_sie(e, sie);
+  base_fill(e, srho, temp, senergy, press, cv, bmod, output, lam, err);
+  if (!(e.mods & MOD_SCALED)) rho = srho;  // an unscaled model fills the caller's rho
+  if (rhoT) sie = mod_energy_out(e, senergy);
 }
 
 } // namespace sgm

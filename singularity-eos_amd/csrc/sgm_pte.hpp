@@ -340,10 +340,19 @@ struct PteBase {
     if (preferred_input(e) & tq::pressure) {
       // IdealGas (modified or not) is the only model here whose preferred
       // input includes pressure: MinimumPressure() = 0,
-      // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549)
-      const Real Pmin = eos_min_pressure(e), Pmax = eos_max_pressure_at_T(e, T, err);
+      // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549), its
+      // closed-form P-T inversion (eos_ideal.hpp:201-207) under the
+      // modifiers' DensityEnergyFromPressureTemperature (shifted_eos.hpp:399-403,
+      // scaled_eos.hpp:328-335), inlined: no call takes this cell's state
+      // by reference
+      const Real Pmin = 0.0, Pmax = 1e100;
       const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
-      eos_rhoe_PT(e, Pguess, T, r, s, lam[m], err);
+      ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
+      if (e.mods & MOD_SHIFTED) s = s + e.shift;
+      if (e.mods & MOD_SCALED) {
+        r = r * e.inv_scale;
+        s = s * e.scale;
+      }
       cv = eos_cv_re(e, r, s, lam[m]);
     } else {
       s = eos_e_rT(e, r, T, lam[m]);

@@ -104,7 +104,12 @@ __global__ void __launch_bounds__(256)
   stage_eos(es, reg + mat);
   EosDev ek = es;
   ek.kind = KIND;
-  const unsigned long in = ~output;
+  // a modified material's FillEos may leave a requested output unwritten (a
+  // ScaledEOS over a (rho, sie) model fills a copy of the energy,
+  // scaled_eos.hpp:155-160): its arrays are read in full so that such an
+  // output keeps the caller's value, as the reference's by-reference
+  // arguments do; the models themselves write every output they are asked for
+  const unsigned long in = ek.mods ? ~0ul : ~output;
   const long nthr = (long)gridDim.x * blockDim.x;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nthr) {
     Real r = (rho && (in & tq::density)) ? rho[i] : 0.0;
