@@ -294,6 +294,27 @@ int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_in
                    double mass_frac_cutoff, const sgm_mix_params *params,
                    int *pte_converged, sgm_stream_t stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). sgm_get_sg_eos with the same arguments and the same
+ * results, which calls no hipMalloc, hipFree or allocating library routine:
+ * its work arrays live in one static device buffer of 64 MiB per GPU (held
+ * until the process ends), laid out for ncell x nmat participating entries
+ * (about 40 B per loop index + 69 B per entry, rounded to 256 B per array:
+ * e.g. 65,536 cells of 6 materials fit). Larger calls are refused with
+ * SGM_E_BUDGET before anything is launched. The closure runs on the plain
+ * per-cell kernels (no batched plan). Asynchronous on `stream` (no
+ * synchronization); every call on one GPU shares that GPU's buffer, so calls
+ * that overlap on one GPU (other streams, other threads, other contexts)
+ * race: order them on one stream. */
+int sgm_get_sg_eos_noalloc(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
+                           const int *eos_offsets, const int *offsets, double *press,
+                           double *pmax, double *vol, double *spvol, double *sie, double *temp,
+                           double *bmod, double *dpde, double *cv, double *frac_mass,
+                           double *frac_vol, double *frac_ie, double *frac_bmod,
+                           double *frac_dpde, double *frac_cv, double mass_frac_cutoff,
+                           const sgm_mix_params *params, int *pte_converged,
+                           sgm_stream_t stream);
 /* No reference counterpart: frees the work buffers and the batched plan that
  * sgm_get_sg_eos keeps in the context between calls (for the 2^20-cell
  * uneven workload about 0.5 GB of straggler records plus the arenas); the

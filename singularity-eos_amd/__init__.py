@@ -117,6 +117,8 @@ _SIGS = {
     "sgm_pte_fixedP": (ci, [vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                             C.POINTER(MixParams), PteStatus, vp]),
     "sgm_set_modifiers": (ci, [vp, ci, vp, vp]),
+    "sgm_get_sg_eos_noalloc": (ci, [vp, ci, ci, ci, ci, vp, vp] + [vp] * 15 +
+                               [cd, C.POINTER(MixParams), vp, vp]),
     "sgm_release_driver_cache": (ci, [vp]),
     "sgm_driver_cache_info": (ci, [vp, vp]),
     "sgm_device_registry": (ci, [vp, C.POINTER(vp), C.POINTER(ci)]),
@@ -325,14 +327,15 @@ SG_FRAC = ("frac_mass", "frac_vol", "frac_ie", "frac_bmod", "frac_dpde", "frac_c
 
 
 def get_sg_eos(ctx, input_mode, eos_offsets, offsets, arrays, cutoff, params=None,
-               converged=None, stream=0):
+               converged=None, stream=0, noalloc=False):
     """sgm_get_sg_eos: eos_offsets is a host sequence of 1-based context
     material indices; offsets (1-based cell indices) and every array in
     `arrays` (SG_FIELDS per cell, SG_FRAC column-major per material, the last
     three optional) are device tensors, updated in place."""
     eo = (C.c_int * len(eos_offsets))(*[int(x) for x in eos_offsets])
     ptrs = [ptr(arrays[k]) for k in SG_FIELDS] + [ptr(arrays.get(k)) for k in SG_FRAC]
-    return lib().sgm_get_sg_eos(ctx.h, len(eos_offsets), int(offsets.numel()),
+    f = lib().sgm_get_sg_eos_noalloc if noalloc else lib().sgm_get_sg_eos
+    return f(ctx.h, len(eos_offsets), int(offsets.numel()),
                                 int(arrays["press"].numel()), int(input_mode), eo, ptr(offsets),
                                 *ptrs, float(cutoff),
                                 C.byref(params) if params is not None else None,

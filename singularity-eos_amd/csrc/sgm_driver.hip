@@ -301,24 +301,19 @@ struct Arena {
   }
 };
 
-}  // namespace
-
-extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
-                              const int *eos_offsets, const int *offsets, double *press,
-                              double *pmax, double *vol, double *spvol, double *sie,
-                              double *temp, double *bmod, double *dpde, double *cv,
-                              double *frac_mass, double *frac_vol, double *frac_ie,
-                              double *frac_bmod, double *frac_dpde, double *frac_cv,
-                              double mass_frac_cutoff, const sgm_mix_params *params,
-                              int *pte_converged, sgm_stream_t stream) {
+// argument checks and the kernels' argument record, shared by both entry points
+int driver_setup(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
+                 const int *eos_offsets, const int *offsets, double *press, double *pmax,
+                 double *vol, double *spvol, double *sie, double *temp, double *bmod,
+                 double *dpde, double *cv, double *frac_mass, double *frac_vol, double *frac_ie,
+                 double *frac_bmod, double *frac_dpde, double *frac_cv,
+                 double mass_frac_cutoff, int *pte_converged, DrvArgs &a) {
   if (!ctx || nmat < 1 || nmat > MAX_DRIVER_MATS || ncell < 0 || cell_dim < ncell ||
       input_int < IN_RHO_T || input_int > IN_RHO_E || !eos_offsets || !offsets || !press ||
       !pmax || !vol || !spvol || !sie || !temp || !bmod || !dpde || !cv || !frac_mass ||
       !frac_vol || !frac_ie || (frac_dpde && !frac_cv))
     return SGM_E_ARG;
   if (ctx->host_only) return SGM_E_NOGPU;
-  if (ncell == 0) return SGM_OK;
-  DrvArgs a;
   a.nmat = nmat;
   a.ncell = ncell;
   a.input = input_int;
@@ -352,6 +347,26 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
   }
   a.d_eos = ctx->d_eos;
   a.nreg = ctx->nmat;
+  return SGM_OK;
+}
+
+}  // namespace
+
+extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, int input_int,
+                              const int *eos_offsets, const int *offsets, double *press,
+                              double *pmax, double *vol, double *spvol, double *sie,
+                              double *temp, double *bmod, double *dpde, double *cv,
+                              double *frac_mass, double *frac_vol, double *frac_ie,
+                              double *frac_bmod, double *frac_dpde, double *frac_cv,
+                              double mass_frac_cutoff, const sgm_mix_params *params,
+                              int *pte_converged, sgm_stream_t stream) {
+  DrvArgs a;
+  const int rc0 = driver_setup(ctx, nmat, ncell, cell_dim, input_int, eos_offsets, offsets,
+                               press, pmax, vol, spvol, sie, temp, bmod, dpde, cv, frac_mass,
+                               frac_vol, frac_ie, frac_bmod, frac_dpde, frac_cv,
+                               mass_frac_cutoff, pte_converged, a);
+  if (rc0 != SGM_OK) return rc0;
+  if (ncell == 0) return SGM_OK;
   hipStream_t s = (hipStream_t)stream;
   // the driver's buffers and plan live in the context (grow-only arenas,
   // reused by the next call); one call at a time per context
@@ -375,8 +390,7 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
     unsigned char *scan_tmp = nullptr;
     ar.take(scan_tmp, scan_bytes + 16);
     if (!ar.ok(s)) return cleanup(SGM_E_HIP);
-    if (hipMemsetAsync(a.npte + ncell, 0, sizeof(int), s) != hipSuccess)
-      return cleanup(SGM_E_HIP);
+    if (launch_zero_ints(a.npte + ncell, 1, s) != hipSuccess) return cleanup(SGM_E_HIP);
     prologue_kernel<<<nb, 256, 0, s>>>(a);
     if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
     if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
@@ -433,4 +447,145 @@ extern "C" int sgm_get_sg_eos(sgm_ctx *ctx, int nmat, int ncell, int cell_dim, i
   epilogue_kernel<<<nb, 256, 0, s>>>(a);
   if (hipGetLastError() != hipSuccess) return cleanup(SGM_E_HIP);
   return cleanup(SGM_OK);
+}
+
+// ---- get_sg_eos without allocation -------------------------------------------
+// sgm_get_sg_eos_noalloc: the driver's work arrays live in one static device
+// buffer per GPU (`scratch`, NOALLOC_SCRATCH_BYTES, held until the process
+// ends), carved per call for the worst case of ncell x nmat participating
+// entries, so no size has to come back to the host; the closure runs on the
+// plain list kernel over the cells of two or more materials (the same
+// results as the batched plan, which would need allocations of its own).
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device: looked up once per
+// device (each GPU holds its own copy of the array) with that device
+// current. Host code uses this address, never the array's name.
+int scratch_address(void **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return SGM_E_HIP;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SGM_E_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SGM_OK;
+}
+
+// 256-byte aligned arrays out of a fixed buffer; ok() once every take fits
+struct FixedArena {
+  unsigned char *base;
+  size_t cap, used = 0;
+  template <class T>
+  void take(T *&p, size_t n) {
+    p = reinterpret_cast<T *>(base + used);
+    used += (sizeof(T) * (n ? n : 1) + 255) / 256 * 256;
+  }
+  bool ok() const { return used <= cap; }
+};
+
+// the loop indices whose cells take part in the closure (two or more
+// materials), appended in any order; range = {0, count} for the list kernel
+__global__ void select_pte_cells_kernel(const int *npte, int ncell, int *cells,
+                                        unsigned long long *range) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= ncell || npte[il] < 2) return;
+  const unsigned long long k = atomicAdd(range + 1, 1ull);
+  cells[k] = il;
+}
+}  // namespace
+
+extern "C" int sgm_get_sg_eos_noalloc(sgm_ctx *ctx, int nmat, int ncell, int cell_dim,
+                                      int input_int, const int *eos_offsets, const int *offsets,
+                                      double *press, double *pmax, double *vol, double *spvol,
+                                      double *sie, double *temp, double *bmod, double *dpde,
+                                      double *cv, double *frac_mass, double *frac_vol,
+                                      double *frac_ie, double *frac_bmod, double *frac_dpde,
+                                      double *frac_cv, double mass_frac_cutoff,
+                                      const sgm_mix_params *params, int *pte_converged,
+                                      sgm_stream_t stream) {
+  DrvArgs a;
+  const int rc0 = driver_setup(ctx, nmat, ncell, cell_dim, input_int, eos_offsets, offsets,
+                               press, pmax, vol, spvol, sie, temp, bmod, dpde, cv, frac_mass,
+                               frac_vol, frac_ie, frac_bmod, frac_dpde, frac_cv,
+                               mass_frac_cutoff, pte_converged, a);
+  if (rc0 != SGM_OK) return rc0;
+  if (ncell == 0) return SGM_OK;
+  const long nnz_max = (long)ncell * nmat;
+  if (nnz_max > 2147483647L) return SGM_E_BUDGET;
+  hipStream_t s = (hipStream_t)stream;
+  size_t scan_bytes = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int *)nullptr, (int *)nullptr,
+                                       ncell + 1, s) != hipSuccess)
+    return SGM_E_HIP;
+  void *base = nullptr;
+  if (scratch_address(&base) != SGM_OK) return SGM_E_HIP;
+  FixedArena ar{static_cast<unsigned char *>(base), NOALLOC_SCRATCH_BYTES};
+  ar.take(a.npte, ncell + 1);
+  ar.take(a.mass_sum, ncell);
+  ar.take(a.eoff, ncell + 1);
+  ar.take(a.cell_in, ncell);
+  ar.take(a.vtot, ncell);
+  unsigned char *scan_tmp = nullptr;
+  ar.take(scan_tmp, scan_bytes + 16);
+  int *cells = nullptr;
+  unsigned long long *range = nullptr;
+  ar.take(cells, ncell);
+  ar.take(range, 2);
+  ar.take(a.emat, nnz_max);
+  ar.take(a.eid, nnz_max);
+  ar.take(a.rho, nnz_max);
+  ar.take(a.vfrac, nnz_max);
+  ar.take(a.esie, nnz_max);
+  ar.take(a.etemp, nnz_max);
+  ar.take(a.epress, nnz_max);
+  ar.take(a.lam, 3L * nnz_max);
+  if (!ar.ok()) return SGM_E_BUDGET;  // nothing launched
+  const int nb = (ncell + 255) / 256;
+  // what must start at zero: the scan's end sentinel and the cell counter
+  if (launch_zero_ints(a.npte + ncell, 1, s) != hipSuccess) return SGM_E_HIP;
+  if (launch_zero_ints(reinterpret_cast<int *>(range), 4, s) != hipSuccess) return SGM_E_HIP;
+  prologue_kernel<<<nb, 256, 0, s>>>(a);
+  if (hipGetLastError() != hipSuccess) return SGM_E_HIP;
+  if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, a.npte, a.eoff, ncell + 1, s) !=
+      hipSuccess)
+    return SGM_E_HIP;
+  fill_kernel<<<nb, 256, 0, s>>>(a);
+  if (hipGetLastError() != hipSuccess) return SGM_E_HIP;
+  if (input_int != IN_P_T) {
+    const int method = input_int == IN_RHO_T   ? PTE_FIXED_T
+                       : input_int == IN_RHO_P ? PTE_FIXED_P
+                                               : PTE_RHOT;
+    select_pte_cells_kernel<<<nb, 256, 0, s>>>(a.npte, ncell, cells, range);
+    if (hipGetLastError() != hipSuccess) return SGM_E_HIP;
+    PteArgs pa;
+    pa.d_eos = a.d_eos;
+    pa.nreg = a.nreg;
+    pa.ncell = ncell;
+    pa.nnz = nnz_max;
+    pa.offsets = a.eoff;
+    pa.mat_ids = a.eid;
+    pa.rho = a.rho;
+    pa.vfrac = a.vfrac;
+    pa.sie = a.esie;
+    pa.temp = a.etemp;
+    pa.press = a.epress;
+    pa.vfrac_tot = a.vtot;
+    pa.sie_tot = a.cell_in;
+    pa.lambda = a.lam;
+    pa.conv = a.conv;
+    if (launch_pte_list(method, pa, to_dev(params), cells, ncell, s,
+                        reinterpret_cast<const long *>(range)) != hipSuccess)
+      return SGM_E_HIP;
+  }
+  epilogue_kernel<<<nb, 256, 0, s>>>(a);
+  return hipGetLastError() == hipSuccess ? SGM_OK : SGM_E_HIP;
 }
