@@ -1093,4 +1093,70 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
   if (rhoT) sie = mod_energy_out(e, senergy);
 }
 
+
+// The EOS calls of the closure solvers, selected at compile time: Ev<true>
+// applies the modifiers (the eos_* functions above), Ev<false> calls the
+// models alone (the base_* functions) for kernels that only ever see
+// unmodified materials, whose hot loops then carry no modifier code.
+template <bool MODS>
+struct Ev;
+template <>
+struct Ev<true> {
+  static SGM_HD unsigned long preferred_input(const EosDev &e) { return sgm::preferred_input(e); }
+  static SGM_HD Real P_rT(const EosDev &e, Real r, Real T, Lam &l) { return eos_P_rT(e, r, T, l); }
+  static SGM_HD Real P_re(const EosDev &e, Real r, Real s, Lam &l) { return eos_P_re(e, r, s, l); }
+  static SGM_HD Real T_re(const EosDev &e, Real r, Real s, Lam &l) { return eos_T_re(e, r, s, l); }
+  static SGM_HD Real cv_rT(const EosDev &e, Real r, Real T, Lam &l) { return eos_cv_rT(e, r, T, l); }
+  static SGM_HD Real cv_re(const EosDev &e, Real r, Real s, Lam &l) { return eos_cv_re(e, r, s, l); }
+  static SGM_HD Real e_rT(const EosDev &e, Real r, Real T, Lam &l) { return eos_e_rT(e, r, T, l); }
+  static SGM_HD Real e_rT_with_P(const EosDev &e, Real r, Real T, Lam &l, Real &P) {
+    return eos_e_rT_with_P(e, r, T, l, P);
+  }
+  static SGM_HD Real e_and_P_rT(const EosDev &e, Real r, Real T, Lam &l, Real &P) {
+    return eos_e_and_P_rT(e, r, T, l, P);
+  }
+  static SGM_HD Real max_density(const EosDev &e) { return eos_max_density(e); }
+  static SGM_HD Real max_pressure_at_T(const EosDev &e, Real T, int &err) {
+    return eos_max_pressure_at_T(e, T, err);
+  }
+  static SGM_HD Real min_pressure(const EosDev &e) { return eos_min_pressure(e); }
+  static SGM_HD Real min_temperature(const EosDev &e) { return eos_min_temperature(e); }
+  static SGM_HD Real rho_pmin(const EosDev &e, Real T) { return eos_rho_pmin(e, T); }
+  static SGM_HD void rhoe_PT(const EosDev &e, Real P, Real T, Real &r, Real &s, Lam &l, int &err) {
+    eos_rhoe_PT(e, P, T, r, s, l, err);
+  }
+  static SGM_HD bool modified(const EosDev &e) { return e.mods != 0; }
+};
+template <>
+struct Ev<false> {
+  static SGM_HD unsigned long preferred_input(const EosDev &e) { return base_preferred_input(e); }
+  static SGM_HD Real P_rT(const EosDev &e, Real r, Real T, Lam &l) { return base_P_rT(e, r, T, l); }
+  static SGM_HD Real P_re(const EosDev &e, Real r, Real s, Lam &l) { return base_P_re(e, r, s, l); }
+  static SGM_HD Real T_re(const EosDev &e, Real r, Real s, Lam &l) { return base_T_re(e, r, s, l); }
+  static SGM_HD Real cv_rT(const EosDev &e, Real r, Real T, Lam &l) { return base_cv_rT(e, r, T, l); }
+  static SGM_HD Real cv_re(const EosDev &e, Real r, Real s, Lam &l) { return base_cv_re(e, r, s, l); }
+  static SGM_HD Real e_rT(const EosDev &e, Real r, Real T, Lam &l) { return base_e_rT(e, r, T, l); }
+  static SGM_HD Real e_rT_with_P(const EosDev &e, Real r, Real T, Lam &l, Real &P) {
+    return base_e_rT_with_P(e, r, T, l, P, r);
+  }
+  static SGM_HD Real e_and_P_rT(const EosDev &e, Real r, Real T, Lam &l, Real &P) {
+    return base_e_and_P_rT(e, r, T, l, P);
+  }
+  static SGM_HD Real max_density(const EosDev &e) { return base_max_density(e); }
+  static SGM_HD Real max_pressure_at_T(const EosDev &e, Real T, int &err) {
+    return base_max_pressure_at_T(e, T, err);
+  }
+  static SGM_HD Real min_pressure(const EosDev &e) { return base_min_pressure(e); }
+  static SGM_HD Real min_temperature(const EosDev &e) { return base_min_temperature(e); }
+  static SGM_HD Real rho_pmin(const EosDev &e, Real T) { return base_rho_pmin(e, T); }
+  static SGM_HD void rhoe_PT(const EosDev &e, Real P, Real T, Real &r, Real &s, Lam &l, int &err) {
+    const RhoSie v = base_rhoe_PT_v(&e, P, T, r, l);
+    r = v.rho;
+    s = v.sie;
+    l = v.lam;
+    err |= v.err;
+  }
+  static SGM_HD bool modified(const EosDev &) { return false; }
+};
+
 } // namespace sgm

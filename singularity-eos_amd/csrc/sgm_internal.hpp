@@ -69,6 +69,14 @@ hipError_t launch_rhoe_PT(const EosDev &e_host, const EosDev *d_eos, int mat, co
 
 // copies the host registry to the device when it changed (sgm_capi.cpp)
 int sync_registry(sgm_ctx *ctx);
+// bit i: registry entry i has modifiers (the batched bins leave its cells to
+// the list kernels)
+inline unsigned long long ctx_mod_mask(const sgm_ctx *ctx) {
+  unsigned long long m = 0;
+  for (int i = 0; i < ctx->nmat && i < 64; ++i)
+    if (ctx->host_eos[i].mods) m |= 1ull << i;
+  return m;
+}
 // MixParams as the kernels take them (NULL = the reference's defaults)
 MixParamsDev to_dev(const sgm_mix_params *p);
 // A batched plan over a CSR topology (sgm_plan_create) whose cells with fewer

@@ -160,8 +160,14 @@ hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 // flag = 0 if a[0..n) == b[0..n) and c[0..m) == d[0..m), else 1
 hipError_t launch_compare_csr(const int *a, const int *b, long n, const uint8_t *c,
                               const uint8_t *d, long m, int *flag, hipStream_t s);
+// sort keys of CSR cells: material count, then the registry ids in cell
+// order; mod_mask (bit i: registry entry i has modifiers) sends the cells of
+// two or more materials that hold a modified material past the
+// register-resident bins (count field | 0x40), to the list kernels, which
+// apply modifiers
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s,
-                            const int *cost = nullptr, int threshold = 0);
+                            const int *cost = nullptr, int threshold = 0,
+                            unsigned long long mod_mask = 0);
 int device_cu_count();
 } // namespace sgm

@@ -35,6 +35,9 @@ struct GraphKey {
 struct sgm_plan {
   sgm_ctx *ctx = nullptr;
   int method = 0;
+  // the context's modified materials when the plan was made: their cells
+  // sit in the list bins (executing after a change of modifiers is refused)
+  unsigned long long mod_mask = 0;
   long ncell = 0, nnz = 0;
   int *d_offsets = nullptr;
   uint8_t *d_mat = nullptr;
@@ -319,7 +322,7 @@ hipError_t sort_cells(sgm_plan *p, const int *cost, int threshold,
   if (e == hipSuccess) e = hipMalloc(&d_ids, sizeof(int) * ncell);
   if (e == hipSuccess)
     e = launch_plan_keys(p->d_offsets, p->d_mat, ncell, p->nnz, d_keys, d_ids, s, cost,
-                         threshold);
+                         threshold, p->mod_mask);
   if (e == hipSuccess)
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys_sorted, d_ids,
                                            p->d_sorted, (int)ncell, 0, 64, s);
@@ -353,6 +356,7 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
   auto *p = new sgm_plan();
   p->ctx = ctx;
   p->method = method;
+  p->mod_mask = ctx_mod_mask(ctx);
   p->ncell = ncell;
   p->nnz = nnz;
   bool ok = hipMalloc(&p->d_offsets, sizeof(int) * (ncell + 1)) == hipSuccess &&
@@ -419,7 +423,9 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
 int plan_match(sgm_plan *p, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
                int method, hipStream_t s, bool &match) {
   match = false;
-  if (!p || p->method != method || p->ncell != ncell || p->nnz != nnz) return SGM_OK;
+  if (!p || p->method != method || p->ncell != ncell || p->nnz != nnz ||
+      p->mod_mask != ctx_mod_mask(p->ctx))
+    return SGM_OK;
   if (!p->d_flag && hipMalloc(&p->d_flag, sizeof(int)) != hipSuccess) return SGM_E_HIP;
   int differ = 1;
   if (launch_compare_csr(p->d_offsets, offsets, ncell + 1, p->d_mat, mat_ids, nnz, p->d_flag,
@@ -666,6 +672,7 @@ int sgm_plan_execute(sgm_plan *p, double *rho, double *vfrac, double *sie, doubl
                      const double *Tnorm, double *lambda, const sgm_mix_params *params,
                      sgm_pte_status status, sgm_stream_t stream) {
   if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot) return SGM_E_ARG;
+  if (p->mod_mask != ctx_mod_mask(p->ctx)) return SGM_E_ARG;  // modifiers changed
   return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,
                         params, status, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
@@ -678,6 +685,7 @@ int sgm_plan_execute_guided(sgm_plan *p, double *rho, double *vfrac, double *sie
   if (!p || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot ||
       !guess_vfrac || !guess_T)
     return SGM_E_ARG;
+  if (p->mod_mask != ctx_mod_mask(p->ctx)) return SGM_E_ARG;  // modifiers changed
   // the guess replaces the volume fractions and Tnorm of the energy-closed
   // formulations; the fixed-T / fixed-P ones have no temperature guess
   if (p->method != PTE_RHOT && p->method != PTE_PT) return SGM_E_ARG;
@@ -716,6 +724,7 @@ int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, doubl
   if (!p || bin < 0 || bin >= (int)p->bins.size() || !rho || !vfrac || !sie || !temp ||
       !press || !vfrac_tot || !sie_tot)
     return SGM_E_ARG;
+  if (p->mod_mask != ctx_mod_mask(p->ctx)) return SGM_E_ARG;  // modifiers changed
   hipStream_t s = (hipStream_t)stream;
   const MixParamsDev P = params_dev(params);
   PteArgs a = make_args(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm, lambda,

@@ -259,8 +259,10 @@ SGM_HD const EosDev &registry_entry(int i) {
 // ---------------------------------------------------------------------------
 // Shared base state (PTESolverBase)
 // ---------------------------------------------------------------------------
-template <int NM, int NEQ>
+template <int NM, int NEQ, bool MODS = true>
 struct PteBase {
+  // the EOS calls: with or without the materials' modifiers
+  using Ev = sgm::Ev<MODS>;
   unsigned char eidx[NM];  // registry index of each material
   SGM_HD const EosDev &E(int m) const { return registry_entry(eidx[m]); }
   Lam lam[NM];
@@ -296,7 +298,7 @@ struct PteBase {
     Real vmax_sum = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      vmax[m] = smin(safety * robust::ratio(rhobar[m], eos_rho_pmin(E(m), T)), 1.0);
+      vmax[m] = smin(safety * robust::ratio(rhobar[m], Ev::rho_pmin(E(m), T)), 1.0);
       if (vfrac[m] <= 0) vfrac[m] = vmax[m];
       vmax_sum += vmax[m];
     }
@@ -325,19 +327,19 @@ struct PteBase {
   // :583-597
   SGM_HD Real PressureFromPreferred(int m, Real r, Real T, Real s) {
     const EosDev &e = E(m);
-    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
-    return eos_P_rT(e, r, T, lam[m]);
+    if (Ev::preferred_input(e) & tq::specific_internal_energy) return Ev::P_re(e, r, s, lam[m]);
+    return Ev::P_rT(e, r, T, lam[m]);
   }
   // the same, with P(rho, T) already evaluated beside e (eos_e_rT_with_P)
   SGM_HD Real PressureFromPreferredP(int m, Real r, Real s, Real P_T) {
     const EosDev &e = E(m);
-    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, lam[m]);
+    if (Ev::preferred_input(e) & tq::specific_internal_energy) return Ev::P_re(e, r, s, lam[m]);
     return P_T;
   }
   // :599-620
   SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
     const EosDev &e = E(m);
-    if (preferred_input(e) & tq::pressure) {
+    if (Ev::preferred_input(e) & tq::pressure) {
       // IdealGas (modified or not) is the only model here whose preferred
       // input includes pressure: MinimumPressure() = 0,
       // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549), its
@@ -348,15 +350,17 @@ struct PteBase {
       const Real Pmin = 0.0, Pmax = 1e100;
       const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
       ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
-      if (e.mods & MOD_SHIFTED) s = s + e.shift;
-      if (e.mods & MOD_SCALED) {
-        r = r * e.inv_scale;
-        s = s * e.scale;
+      if (Ev::modified(e)) {
+        if (e.mods & MOD_SHIFTED) s = s + e.shift;
+        if (e.mods & MOD_SCALED) {
+          r = r * e.inv_scale;
+          s = s * e.scale;
+        }
       }
-      cv = eos_cv_re(e, r, s, lam[m]);
+      cv = Ev::cv_re(e, r, s, lam[m]);
     } else {
-      s = eos_e_rT(e, r, T, lam[m]);
-      cv = eos_cv_rT(e, r, T, lam[m]);
+      s = Ev::e_rT(e, r, T, lam[m]);
+      cv = Ev::cv_rT(e, r, T, lam[m]);
     }
   }
   // :500-581; Tin are the caller's input temperatures
@@ -375,7 +379,7 @@ struct PteBase {
         Real dudt = 0, usum = 0;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-          const Real rho_max = eos_max_density(E(m));
+          const Real rho_max = Ev::max_density(E(m));
           Real s, cv;
           SieCvFromTAndPreferred(m, smin(rho[m], rho_max), press[m], Tguess, s, cv);
           usum += rhobar[m] * s;
@@ -385,7 +389,7 @@ struct PteBase {
         Tguess = smin(P.temperature_limit, smax(Tguess, Tn));
       }
 #pragma unroll
-      for (int m = 0; m < NM; ++m) Tguess = smax(eos_min_temperature(E(m)), Tguess);
+      for (int m = 0; m < NM; ++m) Tguess = smax(Ev::min_temperature(E(m)), Tguess);
       bad = SetVfracFromT(Tguess);
       if (bad) {
         Tguess *= 10.0;
@@ -396,7 +400,7 @@ struct PteBase {
       bad = false;
 #pragma unroll
       for (int m = 0; m < NM; ++m) {
-        if (!bad && rho[m] < eos_rho_pmin(E(m), Tguess)) {
+        if (!bad && rho[m] < Ev::rho_pmin(E(m), Tguess)) {
           bad = true;
           Tguess *= 10.0;
         }
@@ -416,8 +420,8 @@ struct PteBase {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       Real P_T;
-      sie[m] = eos_e_rT_with_P(E(m), rho[m], Tguess, lam[m], P_T);
-      if ((preferred_input(E(m)) & tq::pressure) && !bad_value(press[m])) {
+      sie[m] = Ev::e_rT_with_P(E(m), rho[m], Tguess, lam[m], P_T);
+      if ((Ev::preferred_input(E(m)) & tq::pressure) && !bad_value(press[m])) {
         press[m] = robust::ratio(press[m], uscale);
       } else {
         press[m] = robust::ratio(PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
@@ -472,9 +476,10 @@ struct PteBase {
 // ---------------------------------------------------------------------------
 // PTESolverRhoT (:873-1193). Unknowns (T, alpha_m for m != ms).
 // ---------------------------------------------------------------------------
-template <int NM>
-struct PteRhoT : PteBase<NM, NM> {
-  using B = PteBase<NM, NM>;
+template <int NM, bool MODS = true>
+struct PteRhoT : PteBase<NM, NM, MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = PteBase<NM, NM, MODS>;
   using B::dx;
   using B::E;
   using B::jac;
@@ -579,17 +584,17 @@ struct PteRhoT : PteBase<NM, NM> {
     const Real vf_pert = vf + dv;
     const Real rho_pert = robust::ratio(rb, vf_pert);
     Real P_T;
-    Real e_pert = eos_e_rT_with_P(e, rho_pert, Tcur, l, P_T);
+    Real e_pert = Ev::e_rT_with_P(e, rho_pert, Tcur, l, P_T);
     Real p_pert = robust::ratio(PressureFromPreferredE(e, l, rho_pert, e_pert, P_T), uscale);
     dpdv = smin(robust::ratio((p_pert - pr), dv), -fabs(P.pte_max_dpdv));
     dedv = robust::ratio(rb * robust::ratio(e_pert, uscale) - um, dv);
-    e_pert = eos_e_rT_with_P(e, rh, Tpert, l, P_T);
+    e_pert = Ev::e_rT_with_P(e, rh, Tpert, l, P_T);
     p_pert = robust::ratio(PressureFromPreferredE(e, l, rh, e_pert, P_T), uscale);
     dpdT = robust::ratio((p_pert - pr), dT);
     dedT = robust::ratio(rb * robust::ratio(e_pert, uscale) - um, dT);
   }
   SGM_HD static Real PressureFromPreferredE(const EosDev &e, Lam &l, Real r, Real s, Real P_T) {
-    if (preferred_input(e) & tq::specific_internal_energy) return eos_P_re(e, r, s, l);
+    if (Ev::preferred_input(e) & tq::specific_internal_energy) return Ev::P_re(e, r, s, l);
     return P_T;
   }
   // :1042-1105
@@ -687,7 +692,7 @@ struct PteRhoT : PteBase<NM, NM> {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       const Real rho_min =
-          smax(eos_rho_pmin(E(m), Tnorm * Tequil), eos_rho_pmin(E(m), Tnorm * Tnew));
+          smax(Ev::rho_pmin(E(m), Tnorm * Tequil), Ev::rho_pmin(E(m), Tnorm * Tnew));
       const Real alpha_max = smin(1.0, robust::ratio(rhobar[m], rho_min));
       if (!(alpha_max < vfrac[m])) {
         Real mydx;
@@ -729,7 +734,7 @@ struct PteRhoT : PteBase<NM, NM> {
     for (int m = 0; m < NM; ++m) {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      Real um = rhobar[m] * eos_e_rT_with_P(E(m), rho[m], Tc, lam[m], P_T);
+      Real um = rhobar[m] * Ev::e_rT_with_P(E(m), rho[m], Tc, lam[m], P_T);
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
       press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
@@ -750,9 +755,10 @@ struct PteRhoT : PteBase<NM, NM> {
 // ---------------------------------------------------------------------------
 // PTESolverPT (:1198-1446). Unknowns (T, P); 2x2 system.
 // ---------------------------------------------------------------------------
-template <int NM>
-struct PtePT : PteBase<NM, 2> {
-  using B = PteBase<NM, 2>;
+template <int NM, bool MODS = true>
+struct PtePT : PteBase<NM, 2, MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = PteBase<NM, 2, MODS>;
   using B::dx;
   using B::E;
   using B::err;
@@ -801,7 +807,7 @@ struct PtePT : PteBase<NM, 2> {
     Tequil = 1;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      eos_rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      Ev::rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
       vfrac[m] = robust::ratio(rhobar[m], rho[m]);
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
     }
@@ -823,12 +829,12 @@ struct PtePT : PteBase<NM, 2> {
     for (int m = 0; m < NM; ++m) {
       Real r_pert = rho[m];
       Real e_pert = sie[m];
-      eos_rhoe_PT(E(m), uscale * (Pequil + dp), Tnorm * Tequil, r_pert, e_pert, lam[m], err);
+      Ev::rhoe_PT(E(m), uscale * (Pequil + dp), Tnorm * Tequil, r_pert, e_pert, lam[m], err);
       const Real drdp = robust::ratio(r_pert - rho[m], dp);
       const Real dudp = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dp);
       rbP += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdp;
       dudP += dudp;
-      eos_rhoe_PT(E(m), uscale * Pequil, Tnorm * (Tequil + dT), r_pert, e_pert, lam[m], err);
+      Ev::rhoe_PT(E(m), uscale * Pequil, Tnorm * (Tequil + dT), r_pert, e_pert, lam[m], err);
       const Real drdT = robust::ratio(r_pert - rho[m], dT);
       const Real dudTm = robust::ratio(robust::ratio(rhobar[m] * e_pert, uscale) - u[m], dT);
       rbT += robust::ratio(rhobar[m], rho[m] * rho[m]) * drdT;
@@ -845,14 +851,14 @@ struct PtePT : PteBase<NM, 2> {
     if (scale * dx[0] < -0.95 * Tequil) scale = robust::ratio(-0.95 * Tequil, dx[0]);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const Real Pmin = eos_min_pressure(E(m));
+      const Real Pmin = Ev::min_pressure(E(m));
       scale = smin(fabs(scale),
                    fabs(0.95 * robust::ratio(robust::ratio(Pmin, uscale) - Pequil, dx[1])));
     }
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       const Real Ttest = (Tequil + scale * dx[0]) * Tnorm;
-      const Real Pmax = eos_max_pressure_at_T(E(m), Ttest, err);
+      const Real Pmax = Ev::max_pressure_at_T(E(m), Ttest, err);
       scale = smin(fabs(scale),
                    fabs(0.95 * robust::ratio(robust::ratio(Pmax, uscale) - Pequil, dx[0])));
     }
@@ -870,7 +876,7 @@ struct PtePT : PteBase<NM, 2> {
     Pequil = Ptemp + scale * dx[1];
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      eos_rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
+      Ev::rhoe_PT(E(m), Pequil * uscale, Tequil * Tnorm, rho[m], sie[m], lam[m], err);
       vfrac[m] = robust::ratio(rhobar[m], rho[m]);
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
     }
@@ -892,9 +898,10 @@ struct PtePT : PteBase<NM, 2> {
 // fractions; residual = volume closure + neighbouring pressure differences.
 // The solver never writes temperatures (its Finalize scales them by Tnorm=1).
 // ---------------------------------------------------------------------------
-template <int NM>
-struct PteFixedT : PteBase<NM, NM> {
-  using B = PteBase<NM, NM>;
+template <int NM, bool MODS = true>
+struct PteFixedT : PteBase<NM, NM, MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = PteBase<NM, NM, MODS>;
   using B::dx;
   using B::E;
   using B::jac;
@@ -939,7 +946,7 @@ struct PteFixedT : PteBase<NM, NM> {
     for (int m = 0; m < NM; m++) {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      sie[m] = eos_e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
+      sie[m] = Ev::e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
       uscale += fabs(sie[m]) * rho[m];
       press[m] = P_T;
     }
@@ -979,7 +986,7 @@ struct PteFixedT : PteBase<NM, NM> {
       const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
       const Real vf_pert = vfrac[m] + dv;
       const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
-      const Real p_pert = robust::ratio(eos_P_rT(E(m), rho_pert, Tequil, lam[m]), uscale);
+      const Real p_pert = robust::ratio(Ev::P_rT(E(m), rho_pert, Tequil, lam[m]), uscale);
       dpdv[m] = smin(robust::ratio((p_pert - press[m]), dv), -fabs(P.pte_max_dpdv));
     }
 #pragma unroll
@@ -1001,7 +1008,7 @@ struct PteFixedT : PteBase<NM, NM> {
         scale = robust::ratio(-P.vfrac_safety_fac * vfrac[m], dx[m]);
 #pragma unroll
     for (int m = 0; m < NM; m++) {
-      const Real alpha_max = robust::ratio(rhobar[m], eos_rho_pmin(E(m), Tequil));
+      const Real alpha_max = robust::ratio(rhobar[m], Ev::rho_pmin(E(m), Tequil));
       // already unstable (alpha_max < vfrac): no limit, as the reference
       if (!(alpha_max < vfrac[m]) && scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
         scale = robust::ratio(0.5 * (alpha_max - vfrac[m]), dx[m]);
@@ -1021,7 +1028,7 @@ struct PteFixedT : PteBase<NM, NM> {
       vfrac[m] = vtemp[m] + scale * dx[m];
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      const Real um = rhobar[m] * eos_e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
+      const Real um = rhobar[m] * Ev::e_and_P_rT(E(m), rho[m], Tequil, lam[m], P_T);
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
       press[m] = robust::ratio(P_T, uscale);
@@ -1043,9 +1050,10 @@ struct PteFixedT : PteBase<NM, NM> {
 // and the temperature; residual = each material's pressure against P, then
 // the volume closure.
 // ---------------------------------------------------------------------------
-template <int NM>
-struct PteFixedP : PteBase<NM, NM + 1> {
-  using B = PteBase<NM, NM + 1>;
+template <int NM, bool MODS = true>
+struct PteFixedP : PteBase<NM, NM + 1, MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = PteBase<NM, NM + 1, MODS>;
   using B::dx;
   using B::E;
   using B::jac;
@@ -1083,9 +1091,9 @@ struct PteFixedP : PteBase<NM, NM + 1> {
   // (GetPressureFromPreferred(..., do_e_lookup = true), :583-597)
   SGM_HD Real PressureLookup(int m, Real r, Real T) {
     const EosDev &e = E(m);
-    if (preferred_input(e) & tq::specific_internal_energy)
-      return eos_P_re(e, r, eos_e_rT(e, r, T, lam[m]), lam[m]);
-    return eos_P_rT(e, r, T, lam[m]);
+    if (Ev::preferred_input(e) & tq::specific_internal_energy)
+      return Ev::P_re(e, r, Ev::e_rT(e, r, T, lam[m]), lam[m]);
+    return Ev::P_rT(e, r, T, lam[m]);
   }
   SGM_HD void Residual() {
     NeumaierSum s;
@@ -1105,7 +1113,7 @@ struct PteFixedP : PteBase<NM, NM + 1> {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       Real P_T;
-      sie[m] = eos_e_and_P_rT(E(m), rho[m], Tguess, lam[m], P_T);
+      sie[m] = Ev::e_and_P_rT(E(m), rho[m], Tguess, lam[m], P_T);
       u[m] = robust::ratio(sie[m] * rhobar[m], uscale);
       press[m] = robust::ratio(P_T, uscale);
     }
@@ -1164,7 +1172,7 @@ struct PteFixedP : PteBase<NM, NM + 1> {
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       const Real rho_min =
-          smax(eos_rho_pmin(E(m), Tnorm * Tequil), eos_rho_pmin(E(m), Tnorm * Tnew));
+          smax(Ev::rho_pmin(E(m), Tnorm * Tequil), Ev::rho_pmin(E(m), Tnorm * Tnew));
       const Real alpha_max = robust::ratio(rhobar[m], rho_min);
       if (!(alpha_max < vfrac[m]) && scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
         scale = 0.5 * robust::ratio(alpha_max - vfrac[m], dx[m]);
@@ -1188,7 +1196,7 @@ struct PteFixedP : PteBase<NM, NM + 1> {
       vfrac[m] = vtemp[m] + scale * dx[m];
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      const Real um = rhobar[m] * eos_e_and_P_rT(E(m), rho[m], Tc, lam[m], P_T);
+      const Real um = rhobar[m] * Ev::e_and_P_rT(E(m), rho[m], Tc, lam[m], P_T);
       press[m] = robust::ratio(P_T, uscale);
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
@@ -1215,9 +1223,10 @@ struct PteFixedP : PteBase<NM, NM + 1> {
 // iteration (dpdv, dpde) stays local to Jacobian; dtdv / dtde carry from
 // Jacobian to ScaleDx in the same iteration.
 // ---------------------------------------------------------------------------
-template <int NM>
-struct PteRhoU : PteBase<NM, 2 * NM> {
-  using B = PteBase<NM, 2 * NM>;
+template <int NM, bool MODS = true>
+struct PteRhoU : PteBase<NM, 2 * NM, MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = PteBase<NM, 2 * NM, MODS>;
   using B::dx;
   using B::E;
   using B::jac;
@@ -1300,7 +1309,7 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
       if (!abort) {
         vfrac[m] *= robust::ratio(press[m], (temp[m] * alpha));
         if (Tnorm * Tideal < 0 ||
-            robust::ratio(rhobar[m], vfrac[m]) < eos_rho_pmin(E(m), Tnorm * Tideal)) {
+            robust::ratio(rhobar[m], vfrac[m]) < Ev::rho_pmin(E(m), Tnorm * Tideal)) {
           // restore the volume fractions changed so far and keep the guess
 #pragma unroll
           for (int n = 0; n < NM; n++)
@@ -1314,7 +1323,7 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
     for (int m = 0; m < NM; ++m) {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       Real P_T;
-      const Real sie_m = eos_e_rT_with_P(E(m), rho[m], Tnorm * Tideal, lam[m], P_T);
+      const Real sie_m = Ev::e_rT_with_P(E(m), rho[m], Tnorm * Tideal, lam[m], P_T);
       u[m] = rhobar[m] * robust::ratio(sie_m, uscale);
       press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie_m, P_T), uscale);
     }
@@ -1385,14 +1394,14 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
       const Real dv = (vfrac[m] < 0.5 ? 1.0 : -1.0) * vfrac[m] * P.derivative_eps;
       const Real vf_pert = vfrac[m] + dv;
       const Real rho_pert = robust::ratio(rhobar[m], vf_pert);
-      Real t_pert = robust::ratio(eos_T_re(E(m), rho_pert, sie[m], lam[m]), Tnorm);
+      Real t_pert = robust::ratio(Ev::T_re(E(m), rho_pert, sie[m], lam[m]), Tnorm);
       Real p_pert =
           robust::ratio(B::PressureFromPreferred(m, rho_pert, Tnorm * t_pert, sie[m]), uscale);
       dpdv[m] = smin(robust::ratio(p_pert - press[m], dv), -fabs(P.pte_max_dpdv));
       dtdv[m] = robust::ratio(t_pert - temp[m], dv);
       const Real de = fabs(u[m]) * P.derivative_eps;
       const Real e_pert = robust::ratio(u[m] + de, rhobar[m]);
-      t_pert = robust::ratio(eos_T_re(E(m), rho[m], uscale * e_pert, lam[m]), Tnorm);
+      t_pert = robust::ratio(Ev::T_re(E(m), rho[m], uscale * e_pert, lam[m]), Tnorm);
       p_pert = robust::ratio(
           B::PressureFromPreferred(m, rho[m], Tnorm * t_pert, uscale * e_pert), uscale);
       dpde[m] = robust::ratio(p_pert - press[m], de);
@@ -1437,7 +1446,7 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
       if (scale * dt < -0.1 * temp[m]) scale = -0.1 * robust::ratio(temp[m], dt);
       const Real tt = temp[m] + scale * dt;
       const Real rho_min =
-          smax(eos_rho_pmin(E(m), Tnorm * temp[m]), eos_rho_pmin(E(m), Tnorm * tt));
+          smax(Ev::rho_pmin(E(m), Tnorm * temp[m]), Ev::rho_pmin(E(m), Tnorm * tt));
       const Real alpha_max = robust::ratio(rhobar[m], rho_min);
       if (!(alpha_max < vfrac[m])) {
         if (scale * dx[m] > 0.5 * (alpha_max - vfrac[m]))
@@ -1463,7 +1472,7 @@ struct PteRhoU : PteBase<NM, 2 * NM> {
       rho[m] = robust::ratio(rhobar[m], vfrac[m]);
       u[m] = utemp[m] + scale * dx[NM + m];
       sie[m] = uscale * robust::ratio(u[m], rhobar[m]);
-      temp[m] = robust::ratio(eos_T_re(E(m), rho[m], sie[m], lam[m]), Tnorm);
+      temp[m] = robust::ratio(Ev::T_re(E(m), rho[m], sie[m], lam[m]), Tnorm);
       press[m] =
           robust::ratio(B::PressureFromPreferred(m, rho[m], Tnorm * temp[m], sie[m]), uscale);
     }
@@ -1683,8 +1692,8 @@ SGM_HD inline void skip_to_last_pass(DriverState &d) {
 // P-T inversions start from the previous trial's densities.
 template <class S>
 struct PureTrialSteps : std::true_type {};
-template <int NM>
-struct PureTrialSteps<PtePT<NM>> : std::false_type {};
+template <int NM, bool MODS>
+struct PureTrialSteps<PtePT<NM, MODS>> : std::false_type {};
 
 // The Jacobian of a group's cell: RhoT spreads its per-material derivative
 // evaluations over the group's lanes; the other formulations evaluate it on
@@ -1693,8 +1702,8 @@ template <int G, class S>
 __device__ __forceinline__ void group_jacobian(S &s, int, int) {
   s.Jacobian();
 }
-template <int G, int NM>
-__device__ __forceinline__ void group_jacobian(PteRhoT<NM> &s, int g, int base) {
+template <int G, int NM, bool MODS>
+__device__ __forceinline__ void group_jacobian(PteRhoT<NM, MODS> &s, int g, int base) {
   if constexpr (G >= NM)
     s.template JacobianGroup<G>(g, base);
   else

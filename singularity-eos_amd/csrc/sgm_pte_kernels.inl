@@ -32,16 +32,20 @@ __device__ __forceinline__ void stage_registry(EosDev *lds, const EosDev *reg, i
   __syncthreads();
 }
 
-// closure formulation by method id: 0 RhoT, 1 PT, 2 FixedT, 3 FixedP, 4 RhoU
-template <int NM, int METHOD>
+// closure formulation by method id: 0 RhoT, 1 PT, 2 FixedT, 3 FixedP, 4 RhoU.
+// MODS: whether the EOS calls apply the materials' modifiers. The batched
+// bin and tail kernels are built without (the plans and the server send
+// cells with a modified material to the list kernel), so their hot loops
+// carry no modifier code; the plain and list kernels are built with.
+template <int NM, int METHOD, bool MODS = true>
 using SysOf = typename std::conditional<
-    METHOD == 0, PteRhoT<NM>,
+    METHOD == 0, PteRhoT<NM, MODS>,
     typename std::conditional<
-        METHOD == 1, PtePT<NM>,
+        METHOD == 1, PtePT<NM, MODS>,
         typename std::conditional<
-            METHOD == 2, PteFixedT<NM>,
-            typename std::conditional<METHOD == 3, PteFixedP<NM>, PteRhoU<NM>>::type>::type>::
-        type>::type;
+            METHOD == 2, PteFixedT<NM, MODS>,
+            typename std::conditional<METHOD == 3, PteFixedP<NM, MODS>,
+                                      PteRhoU<NM, MODS>>::type>::type>::type>::type;
 
 // formulations whose materials keep temperatures of their own (RhoU)
 template <class S, class = void>
@@ -242,7 +246,7 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   if (chunk < 64) chunk = 64;  // one claim must cover a whole wave
-  SysOf<NM, METHOD> s;
+  SysOf<NM, METHOD, false> s;
   DriverState d;
   CellStatus st;
   long c = -1, jpos = -1;
@@ -373,7 +377,7 @@ __attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
     const long c = pk.cell_of[slot];
     if (c < 0) continue;  // finished in level 1
     const int off = a.offsets[c];
-    SysOf<NM, METHOD> s;
+    SysOf<NM, METHOD, false> s;
     DriverState d;
     CellStatus st;
     // the record, free once loaded, keeps the state before each pass for the
@@ -433,7 +437,8 @@ hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int
   const ParkArgs pk = park ? *park : ParkArgs{};
 #define SGM_BIN(K)                                                                             \
   case K:                                                                                      \
-    static_assert(ParkWordCount<SysOf<K, METHOD>>() <= ParkWords(K, K + 1), "park record");    \
+    static_assert(ParkWordCount<SysOf<K, METHOD, false>>() <= ParkWords(K, K + 1),             \
+                  "park record");                                                              \
     if (pk.after > 0 && pk.stride < ParkWords(K, K + 1)) return hipErrorInvalidValue;          \
     pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range,    \
                                                        pk);                                    \

@@ -14,16 +14,21 @@ __global__ void zero_ints_kernel(int *p, int n) {
 // key = nmat << 56 | material ids in cell order (8 bits each; ids < 64, so bit
 // 55 is free). With a cost array, bit 55 is set on cells below the
 // threshold: within a material count, the costly cells sort first.
+// Cells of two or more materials that hold a material with modifiers get
+// count field nm | 0x40: past the register-resident bins.
 __global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long ncell, long nnz,
-                                 const int *cost, int threshold, unsigned long long *keys,
-                                 int *ids) {
+                                 const int *cost, int threshold, unsigned long long mod_mask,
+                                 unsigned long long *keys, int *ids) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const long o0 = offsets[c], o1 = offsets[c + 1];
   unsigned long long k;
   if (o0 >= 0 && o1 >= o0 && o1 <= nnz && o1 - o0 <= MAX_MATS_PER_CELL) {
     const int nm = (int)(o1 - o0);
-    k = (unsigned long long)nm << 56;
+    bool modified = false;
+    for (int m = 0; m < nm && mod_mask; ++m)
+      modified = modified || (mat[o0 + m] < 64 && ((mod_mask >> mat[o0 + m]) & 1ull));
+    k = (unsigned long long)((nm >= 2 && modified) ? (nm | 0x40) : nm) << 56;
     for (int m = 0; m < nm && m < 7; ++m) k |= (unsigned long long)mat[o0 + m] << (48 - 8 * m);
     if (cost && cost[c] < threshold) k |= 1ull << 55;
   } else {
@@ -65,10 +70,10 @@ hipError_t launch_compare_csr(const int *a, const int *b, long n, const uint8_t 
 
 hipError_t launch_plan_keys(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                             unsigned long long *keys, int *ids, hipStream_t s, const int *cost,
-                            int threshold) {
+                            int threshold, unsigned long long mod_mask) {
   if (ncell <= 0) return hipSuccess;
   plan_keys_kernel<<<(int)((ncell + 255) / 256), 256, 0, s>>>(offsets, mat, ncell, nnz, cost,
-                                                                threshold, keys, ids);
+                                                                threshold, mod_mask, keys, ids);
   return hipGetLastError();
 }
 

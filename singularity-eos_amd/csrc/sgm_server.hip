@@ -323,7 +323,8 @@ int launch_group(sgm_server *sv, Lane &ln, const Layout &L) {
       int *ids = (int *)D(L.ids), *sorted = (int *)D(L.sorted);
       long *range = (long *)D(L.range);
       int *counters = (int *)D(L.counters);
-      ok(launch_plan_keys(off, a.mat_ids, L.ncell, L.nnz, keys, ids, s));
+      ok(launch_plan_keys(off, a.mat_ids, L.ncell, L.nnz, keys, ids, s, nullptr, 0,
+                          ctx_mod_mask(ctx)));
       size_t tb = L.sort_tmp_bytes;
       ok(hipcub::DeviceRadixSort::SortPairs(D(L.sort_tmp), tb, keys, keys2, ids, sorted,
                                             (int)L.ncell, 0, 64, s));

@@ -129,11 +129,15 @@ __global__ void __launch_bounds__(256)
     eos_fill(ek, r, t, s, p, c, b, output, lam, err);
     // an output set the model cannot fill (the reference's EOS_ERROR) writes
     // nothing: the point's arrays keep their values
+    // (a modified material's FillEos may also change an input in place, as
+    // ScaledEOS scales the energy of a (rho, T) model, scaled_eos.hpp:150-153:
+    // every array it was given is written back)
     const bool filled = !(err & ERRB_FILLEOS);
-    if (filled && rho && (output & tq::density)) rho[i] = r;
-    if (filled && temp && (output & tq::temperature)) temp[i] = t;
-    if (filled && sie && (output & tq::specific_internal_energy)) sie[i] = s;
-    if (filled && press && (output & tq::pressure)) press[i] = p;
+    const unsigned long wr = ek.mods ? ~0ul : output;
+    if (filled && rho && (wr & tq::density)) rho[i] = r;
+    if (filled && temp && (wr & tq::temperature)) temp[i] = t;
+    if (filled && sie && (wr & tq::specific_internal_energy)) sie[i] = s;
+    if (filled && press && (wr & tq::pressure)) press[i] = p;
     if (filled && cv && (output & tq::specific_heat)) cv[i] = c;
     if (filled && bmod && (output & tq::bulk_modulus)) bmod[i] = b;
     if (lp) {

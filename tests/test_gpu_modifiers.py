@@ -92,9 +92,8 @@ def test_modified_materials_in_the_closure(sgm, workloads, oracle, torch_gpu, me
     ctx, t = wl.pool_context()
     o, otab = oracle_pool(oracle, wl, t)
     # scaled ideal gas, shifted stiffened gas, scaled+shifted Gruneisen,
-    # shifted Vinet, scaled table
-    plan = {0: MODS[0], 2: MODS[1], 3: MODS[2], 4: dict(shifted=True, shift=-3.0e8),
-            5: dict(scaled=True, scale=1.05)}
+    # shifted Vinet; the second ideal gas and the table unmodified
+    plan = {0: MODS[0], 2: MODS[1], 3: MODS[2], 4: dict(shifted=True, shift=-3.0e8)}
     a = wl.uneven(ctx, 8192 if method == 0 else 4096, seed=0x5EED0000 + 51)
     for m, mod in plan.items():
         ctx.set_modifiers(m, **mod)
@@ -105,6 +104,31 @@ def test_modified_materials_in_the_closure(sgm, workloads, oracle, torch_gpu, me
     assert_identical(g, r, f"modified closure m{method}")
     assert np.array_equal(g["err"], r["err"])
     assert g["conv"].mean() > 0.8
-    b, _ = run_batched(torch_gpu, sgm, ctx, a, method, params)
+    b, info = run_batched(torch_gpu, sgm, ctx, a, method, params)
     assert_identical(b, g, "modified batched vs plain")
+    # the batched kernels are built without modifiers: cells with a modified
+    # material run on the list kernel, the others in the bins
+    listed = sum(i["ncells"] for i in info if i["nmat"] == 0)
+    binned = sum(i["ncells"] for i in info if 2 <= i["nmat"] <= 6)
+    assert listed > 0 and binned > 0 and listed + binned == a["ncell"]
+    ctx.close()
+
+
+def test_plan_refuses_execution_after_modifiers_change(sgm, workloads, torch_gpu):
+    torch = torch_gpu
+    wl = workloads
+    ctx, _ = wl.pool_context()
+    a = wl.uneven(ctx, 2048, seed=0x5EED0000 + 52)
+    from test_gpu_pte import dev
+    d = dev(torch, a)
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], 0)
+    args = (d["rho"], d["vfrac"], d["sie"], d["temp"], d["press"], d["vfrac_tot"], d["sie_tot"])
+    assert plan.execute(*args) == 0
+    ctx.set_modifiers(3, shifted=True, shift=1e8)
+    assert plan.execute(*args) == -1
+    plan.close()
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], 0)
+    assert plan.execute(*args) == 0
+    torch.cuda.synchronize()
+    plan.close()
     ctx.close()
