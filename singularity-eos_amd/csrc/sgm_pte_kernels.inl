@@ -336,20 +336,24 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
 // many cells that stall at a fixed point or finish soon keep the throughput
 // of one lane each; the few that wander for hundreds of iterations get the
 // shorter iterations of a group.
-// The 5- and 6-material tails carry the longest chains (iteration caps of
-// 640 and 768) and run one wave per SIMD so their state stays in registers;
-// the others run two.
+// Waves per SIMD (WPE): level 1, which resumes many parked cells at once,
+// runs two for 2..4 materials (latency hidden by the other wave) and one
+// for 5 and 6; level 2, a few cells that wander for hundreds of iterations
+// with nothing else on the GPU to hide their latency, always runs one, so
+// that its state stays in registers (the register file's second half takes
+// what the first cannot hold) instead of going to scratch memory.
 #ifndef SGM_TAIL_WAVES_PER_EU_LARGE
 #define SGM_TAIL_WAVES_PER_EU_LARGE 1
 #endif
-template <int NM, int METHOD>
+template <int NM>
+constexpr int TailWavesL1() {
+  return NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2;
+}
+template <int NM, int METHOD, int WPE>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
-                                   NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2))) pte_tail_kernel(PteArgs a,
-                                                                      MixParamsDev P,
-                                                                      ParkArgs pk,
-                                                                      int level,
-                                                                      int resident) {
+__attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixParamsDev P,
+                                                                ParkArgs pk, int level,
+                                                                int resident) {
   constexpr int G = TailGroup(METHOD, NM);
   constexpr int per_slot = 64 / G;  // records per wave slot
   extern __shared__ EosDev lds[];
@@ -388,9 +392,16 @@ __attribute__((amdgpu_waves_per_eu(NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2,
     bool moved = false;
     if constexpr (G > 1) {
       if (grp) {
+        // the group's probe copy in LDS: a pass's comparison and store then
+        // cost LDS latency instead of a round trip to memory
+        constexpr int PW = ParkWordCount<SysOf<NM, METHOD, false>>();
+        __shared__ double probe_lds[64 / G][PW];
+        double *pw = probe_lds[lane / G];
+        if (g == 0)  // the one lane that probes
+          for (int k = 0; k < PW; ++k) pw[k] = rec[k];
         while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
           int same = 0;
-          if (g == 0) same = fixed_point_probe(s, d, st, rec) ? 1 : 0;
+          if (g == 0) same = fixed_point_probe(s, d, st, pw) ? 1 : 0;
           if (__shfl(same, base, 64)) skip_to_last_pass(d);
         }
       }
@@ -474,14 +485,22 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   // other); empty slots exit at once
   for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
     const int gl = level == 1 ? g : pk.nwaves;
+#define SGM_TAIL(K)                                                                            \
+  case K:                                                                                      \
+    if (level == 1)                                                                            \
+      pte_tail_kernel<K, METHOD, TailWavesL1<K>()><<<gl, 64, lds, s>>>(a, P, pk, level, cap);  \
+    else                                                                                       \
+      pte_tail_kernel<K, METHOD, 1><<<gl, 64, lds, s>>>(a, P, pk, level, 4 * ncu);             \
+    break;
     switch (nm) {
-    case 2: pte_tail_kernel<2, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
-    case 3: pte_tail_kernel<3, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
-    case 4: pte_tail_kernel<4, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
-    case 5: pte_tail_kernel<5, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
-    case 6: pte_tail_kernel<6, METHOD><<<gl, 64, lds, s>>>(a, P, pk, level, cap); break;
+      SGM_TAIL(2)
+      SGM_TAIL(3)
+      SGM_TAIL(4)
+      SGM_TAIL(5)
+      SGM_TAIL(6)
     default: return hipErrorInvalidValue;
     }
+#undef SGM_TAIL
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
