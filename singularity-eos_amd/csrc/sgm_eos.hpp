@@ -963,7 +963,9 @@ SGM_HD void base_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &p
 // int more than the model's own
 SGM_HD int mods_of(const EosDev &e) {
   int m = e.mods;
+#if defined(__HIP_DEVICE_COMPILE__)
   asm volatile("" : "+v"(m));
+#endif
   return m;
 }
 SGM_HD Real mod_rho(const EosDev &e, Real rho) {

@@ -56,6 +56,7 @@ struct ParkArgs {
   // second, where TailGroup lanes share each cell
   int after2 = 0;                   // 0: one level
   int *l2_count = nullptr;          // cells saved for the second level
+  int *l2_next = nullptr;           // second level (RhoT): next record item to take
 };
 
 // Closure formulations, by the method ids of the C ABI.

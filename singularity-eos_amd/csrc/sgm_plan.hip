@@ -49,7 +49,7 @@ struct sgm_plan {
   int *d_run_of = nullptr, *d_run_base = nullptr, *d_run_cap = nullptr;
   int *d_run_count = nullptr, *d_wave_run = nullptr, *d_wave_first = nullptr;
   int nruns = 0;
-  int *d_l2_counts = nullptr;    // per bin: cells saved for the tail's second level
+  int *d_l2_counts = nullptr;    // per bin: cells saved for the tail's second level, its queue
   int *d_flag = nullptr;         // plan_match's result
   int park_after = 20;           // Newton iterations before a cell is parked; 0 = never
   std::vector<Bin> bins;
@@ -152,7 +152,7 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
   if (e == hipSuccess) e = up(&p->d_wave_run, wave_run);
   if (e == hipSuccess) e = up(&p->d_wave_first, wave_first);
   if (e == hipSuccess)
-    e = hipMalloc(&p->d_l2_counts, sizeof(int) * std::max<size_t>(1, p->bins.size()));
+    e = hipMalloc(&p->d_l2_counts, sizeof(int) * 2 * std::max<size_t>(1, p->bins.size()));
   for (size_t bi = 0; bi < p->bins.size() && e == hipSuccess; ++bi) {
     Bin &b = p->bins[bi];
     if (b.nm < 2 || b.nm > 6 || nrec[bi] == 0) continue;
@@ -160,7 +160,8 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
     k.stride = ParkWords(b.nm, b.nm + 1);
     e = hipMalloc(&k.rec, sizeof(double) * k.stride * (size_t)nrec[bi]);
     if (e == hipSuccess) e = hipMalloc(&k.cell_of, sizeof(int) * nrec[bi]);
-    k.l2_count = p->d_l2_counts + bi;
+    k.l2_count = p->d_l2_counts + 2 * bi;
+    k.l2_next = k.l2_count + 1;
     k.after2 = 4;
     k.run_of = p->d_run_of + b.start;
     k.run_base = p->d_run_base + run0[bi];
@@ -196,7 +197,7 @@ hipError_t reset_counters(sgm_plan *p, hipStream_t s) {
   if (e == hipSuccess && p->park_after > 0 && p->nruns > 0)
     e = launch_zero_ints(p->d_run_count, p->nruns, s);
   if (e == hipSuccess && p->park_after > 0 && p->d_l2_counts)
-    e = launch_zero_ints(p->d_l2_counts, (int)p->bins.size(), s);
+    e = launch_zero_ints(p->d_l2_counts, 2 * (int)p->bins.size(), s);
   return e;
 }
 

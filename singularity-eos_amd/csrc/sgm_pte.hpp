@@ -654,6 +654,56 @@ struct PteRhoT : PteBase<NM, NM, MODS> {
     }
     AssembleJacobian(dpdv, dedv, dpdT, dedT_sum);
   }
+  // Jacobian() by a whole wave holding identical copies of the cell
+  // (pte_iterate_wide): lane 8 t + m evaluates material m's volume-fraction
+  // derivatives (t even) or temperature derivatives (t odd), the two fused
+  // e/P evaluations of MaterialDerivs, written as one code path so that the
+  // wave runs each material kind's evaluation once. Every lane then gathers
+  // the derivatives in material order and each lambda cache from the
+  // temperature lane, whose evaluation MaterialDerivs makes last (the
+  // evaluations write the caches and never read them). The values, the order
+  // of the dedT sum and the caches are those of Jacobian().
+  __device__ __forceinline__ void JacobianWide(int lane) {
+    const Real Tcur = Tnorm * Tequil;
+    const Real dT = Tequil * P.derivative_eps;
+    const Real Tpert = Tnorm * (Tequil + dT);
+    const int m = (lane & 7) < NM ? (lane & 7) : NM - 1;
+    const bool tpart = (lane & 8) != 0;
+    int ei = B::eidx[0];
+    Lam l = lam[0];
+#pragma unroll
+    for (int k = 1; k < NM; ++k) {
+      ei = (m == k) ? (int)B::eidx[k] : ei;
+      l.lRho = sel(m == k, lam[k].lRho, l.lRho);
+      l.lT = sel(m == k, lam[k].lT, l.lT);
+    }
+    const EosDev &e = registry_entry(ei);
+    const Real vf = pick<NM>(vfrac, m), rb = pick<NM>(rhobar, m), rh = pick<NM>(rho, m),
+               pr = pick<NM>(press, m), um = pick<NM>(u, m);
+    const Real dv = (vf < 0.5 ? 1.0 : -1.0) * vf * P.derivative_eps;
+    const Real rho_pert = robust::ratio(rb, vf + dv);
+    const Real rx = tpart ? rh : rho_pert;
+    const Real Tx = tpart ? Tpert : Tcur;
+    const Real delta = tpart ? dT : dv;
+    Real P_T;
+    const Real e_pert = Ev::e_rT_with_P(e, rx, Tx, l, P_T);
+    const Real p_pert = robust::ratio(PressureFromPreferredE(e, l, rx, e_pert, P_T), uscale);
+    Real dp = robust::ratio((p_pert - pr), delta);
+    if (!tpart) dp = smin(dp, -fabs(P.pte_max_dpdv));
+    const Real de = robust::ratio(rb * robust::ratio(e_pert, uscale) - um, delta);
+    Real dpdv[NM], dedv[NM], dpdT[NM];
+    Real dedT_sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      dpdv[k] = __shfl(dp, k, 64);
+      dedv[k] = __shfl(de, k, 64);
+      dpdT[k] = __shfl(dp, 8 + k, 64);
+      dedT_sum += __shfl(de, 8 + k, 64);
+      lam[k].lRho = __shfl(l.lRho, 8 + k, 64);
+      lam[k].lT = __shfl(l.lT, 8 + k, 64);
+    }
+    AssembleJacobian(dpdv, dedv, dpdT, dedT_sum);
+  }
   SGM_HD void AssembleJacobian(const Real (&dpdv)[NM], const Real (&dedv)[NM],
                                const Real (&dpdT)[NM], Real dedT_sum) {
     const Real dedv_s = pick<NM>(dedv, ms), dpdT_s = pick<NM>(dpdT, ms),
@@ -738,6 +788,61 @@ struct PteRhoT : PteBase<NM, NM, MODS> {
       sie[m] = robust::ratio(um, rhobar[m]);
       u[m] = robust::ratio(um, uscale);
       press[m] = robust::ratio(B::PressureFromPreferredP(m, rho[m], sie[m], P_T), uscale);
+    }
+    Residual();
+    return B::ResidualNorm();
+  }
+  // TestUpdate by a whole wave (pte_iterate_wide): lane 8 t + m takes the
+  // trial's new temperature and volume fractions like every lane, then
+  // evaluates material m only; the eight lanes of each t then gather the
+  // materials' densities, energies, pressures and lambda caches in material
+  // order and form the residual. Each group of eight lanes can run a trial
+  // of its own scale. The values, the order of every sum and the caches are
+  // those of TestUpdate.
+  __device__ __forceinline__ Real TestUpdateWide(Real scale, bool cache_state, int lane) {
+    if (cache_state) {
+      Ttemp = Tequil;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) vtemp[m] = vfrac[m];
+    }
+    Tequil = Ttemp + scale * dx[0];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const Real dxm = sel(m < ms, dx[m + 1], dx[m]);
+      if (m != ms) vfrac[m] = vtemp[m] + scale * dxm;
+    }
+    NeumaierSum s;
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      if (m != ms) s.add(vfrac[m]);
+    put<NM>(vfrac, ms, 1 - s.result());
+    const Real Tc = Tnorm * Tequil;
+    const int m = (lane & 7) < NM ? (lane & 7) : NM - 1;
+    int ei = B::eidx[0];
+    Lam l = lam[0];
+#pragma unroll
+    for (int k = 1; k < NM; ++k) {
+      ei = (m == k) ? (int)B::eidx[k] : ei;
+      l.lRho = sel(m == k, lam[k].lRho, l.lRho);
+      l.lT = sel(m == k, lam[k].lT, l.lT);
+    }
+    const EosDev &e = registry_entry(ei);
+    const Real rb = pick<NM>(rhobar, m);
+    const Real rh = robust::ratio(rb, pick<NM>(vfrac, m));
+    Real P_T;
+    const Real um = rb * Ev::e_rT_with_P(e, rh, Tc, l, P_T);
+    const Real se = robust::ratio(um, rb);
+    const Real us = robust::ratio(um, uscale);
+    const Real pr = robust::ratio(PressureFromPreferredE(e, l, rh, se, P_T), uscale);
+    const int b = lane & ~7;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      rho[k] = __shfl(rh, b + k, 64);
+      sie[k] = __shfl(se, b + k, 64);
+      u[k] = __shfl(us, b + k, 64);
+      press[k] = __shfl(pr, b + k, 64);
+      lam[k].lRho = __shfl(l.lRho, b + k, 64);
+      lam[k].lT = __shfl(l.lT, b + k, 64);
     }
     Residual();
     return B::ResidualNorm();
@@ -1682,7 +1787,7 @@ SGM_HD bool fixed_point_probe(Sys &s, DriverState &d, CellStatus &st, double *w)
   return f.same;
 }
 // after a pass whose state matched the probe copy: resume at the last pass
-SGM_HD inline void skip_to_last_pass(DriverState &d) {
+SGM_HD void skip_to_last_pass(DriverState &d) {
   if (d.niter < d.max_iter - 1) d.niter = d.max_iter - 1;
 }
 
@@ -1798,6 +1903,139 @@ __device__ __forceinline__ bool pte_iterate_group(S &s, DriverState &d, CellStat
   ++d.niter;
   return d.niter >= d.max_iter;
 }
+
+// One pass of the PTESolver loop body by a whole wave whose 64 lanes hold
+// identical copies of one RhoT cell, lane = 8 t + m: the Jacobian's
+// derivative evaluations and every trial's material evaluations run one
+// material per lane (JacobianWide, TestUpdateWide), so the wave evaluates
+// each material kind once per stage instead of each material in turn; the
+// full and the half step run as t = 0 and t = 1, then the backtracking
+// scales as t = 0 .. line_search_max_iter - 1. The accepted trial is the one
+// the sequential loop would stop at, and every lane takes its state, so the
+// result is bit-identical to pte_iterate. Needs line_search_max_iter <= 8
+// and NM <= 8, and a converged wave (every lane active); every branch
+// depends on values that are equal on all lanes.
+template <int NM, class S>
+__device__ __forceinline__ bool pte_iterate_wide(S &s, DriverState &d, CellStatus &st, int lane) {
+  const MixParamsDev &p = s.P;
+  const int t = lane >> 3;
+  if (d.niter >= d.max_iter) return true;
+  st.niter = d.niter;
+  s.CheckPTE(d.converged, d.close_enough);
+  if (d.converged) return true;
+  s.JacobianWide(lane);
+  if (!s.Solve()) {
+    d.converged = false;
+    s.err |= ERRB_LINSOLVE;
+    return true;
+  }
+  Real scale = smin(1.0, s.ScaleDx());
+  if (fabs(scale) < p.pte_small_step_thresh) d.small_step_iters++;
+  if (d.small_step_iters >= p.pte_small_step_tries) {
+    d.converged = false;
+    s.err |= ERRB_SMALLSTEP;
+    return true;
+  }
+  const Real gradfdx = -2.0 * scale * d.err;
+  const Real err_old = d.err;
+  // The two trial stages around one (rolled) TestUpdateWide. Stage 0: the
+  // full step (t = 1: the half step). Stage 1, when the full step does not
+  // decrease the residual enough: lanes of t the t-th scale of the
+  // backtracking loop (scales past line_search_max_iter are computed and
+  // unused; every lane takes part in the gathers).
+  Real err_full = 0, err_mid = 0, sc0 = 0.5;
+  bool search = false;
+  Real ks[8];
+#pragma clang loop unroll(disable)
+  for (int stage = 0; stage < 2; ++stage) {
+    Real sk = t == 1 ? 0.5 : 1.0;
+    if (stage == 1) {
+      sk = sc0;
+      for (int k = 0; k < t; ++k) sk *= p.line_search_fac;
+    }
+    const Real e = s.TestUpdateWide(sk, stage == 0, lane);
+    if (stage == 1) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) ks[k] = __shfl(e, 8 * k, 64);
+      break;
+    }
+    err_full = __shfl(e, 0, 64);
+    err_mid = __shfl(e, 8, 64);
+    // a scalar: the loop's exit is uniform
+    search = __builtin_amdgcn_readfirstlane(
+        err_full > err_old + p.line_search_alpha * gradfdx ? 1 : 0);
+    if (err_mid < err_full && err_mid < err_old)
+      sc0 = 0.75 + 0.5 * robust::ratio(err_mid - err_full, err_full - 2.0 * err_mid + err_old);
+    if (!(search && p.line_search_max_iter > 0)) break;
+  }
+  int src = 0;
+  Real err = err_full;
+  if (search) {
+    if (p.line_search_max_iter <= 0) {
+      src = 1;  // the state of the half step stays, with the full step's error
+    } else {
+      int chosen = p.line_search_max_iter - 1;
+      Real sc = sc0;
+      bool found = false;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (!found && k < p.line_search_max_iter) {
+          if (ks[k] < err_old + p.line_search_alpha * sc * gradfdx) {
+            chosen = k;
+            found = true;
+          }
+          sc *= p.line_search_fac;
+        }
+      }
+      src = chosen;
+      err = pick<8>(ks, chosen);
+      const int failed = found ? chosen - 1 : p.line_search_max_iter - 1;
+      if (failed > d.max_line_niter) d.max_line_niter = failed;
+    }
+  }
+  group_take(s, d, st, 8 * src);
+  d.err = err;
+  s.Fixup();
+  if (((err > p.pte_slow_convergence_thresh * err_old) || d.niter == d.max_iter - 1) &&
+      d.close_enough) {
+    d.converged = true;
+    d.slow = true;
+    return true;
+  }
+  ++d.niter;
+  return d.niter >= d.max_iter;
+}
+
+// When a loop that probes for fixed points (fixed_point_probe) needs the
+// full comparison: a pass can only have repeated the state when it left the
+// residual norm d.err unchanged, which wandering cells almost never do. After
+// each pass q, ProbeGate::due is true when d.err equals the previous pass's;
+// only then does the loop run the probe (compare with the copy, then store
+// the state as the new copy), and it may skip when the probe matches and the
+// copy was stored after pass q - 1 (a copy from an earlier pass would find a
+// periodic orbit, not a fixed point). A fixed point is then found one pass
+// after the first pass that repeats d.err, and the copy is written only at
+// those passes instead of after every pass.
+struct ProbeGate {
+  unsigned long long last = 0;  // d.err of the state after the previous pass
+  int copy_at = 0;              // pass whose state the copy holds (0: the state at load)
+  int q = 0;                    // passes run
+  SGM_HD void start(Real err) { last = __builtin_bit_cast(unsigned long long, err); }
+  // after a pass: whether to run the probe
+  SGM_HD bool due(Real err) {
+    ++q;
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, err);
+    const bool eq = b == last;
+    last = b;
+    return eq;
+  }
+  // after the probe that due() asked for: whether the loop may skip
+  SGM_HD bool skip(bool same) {
+    const bool ok = same && copy_at == q - 1;
+    copy_at = q;
+    return ok;
+  }
+};
 
 template <int NM, class S>
 SGM_HD void pte_end(S &s, const DriverState &d, CellStatus &st) {

@@ -324,24 +324,26 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
   }
 }
 
-// Resumes the parked cells of one bin in two levels; a persistent grid of
-// one-wave blocks walks the bin's tail wave slots (each on records of a
-// single signature run) in both. Level 1: one lane per cell; a cell that
+// Resumes the parked cells of one bin in two levels (LEVEL). Level 1: a
+// persistent grid of one-wave blocks walks the bin's tail wave slots (each
+// on records of a single signature run), one lane per cell; a cell that
 // reaches a fixed point of the Newton loop skips to its last pass, and a
 // cell still iterating pk.after2 iterations later is saved back into its
-// record for level 2 (finished cells clear their record's cell id). Level 2:
-// TailGroup lanes (G) per remaining cell, which run its line-search trials
-// side by side (pte_iterate_group), when those cells fit the grid at once
-// (else one lane each). Every cell runs to the end of its Newton loop. The
-// many cells that stall at a fixed point or finish soon keep the throughput
-// of one lane each; the few that wander for hundreds of iterations get the
-// shorter iterations of a group.
+// record for level 2 (finished cells clear their record's cell id). Level 2
+// finishes the few cells that wander for hundreds of iterations, where each
+// iteration's latency is what counts:
+//  * RhoT: one wave per cell (tail_wide), cells taken in record order from
+//    a queue, so a wave that finishes a short cell takes the next one;
+//  * FixedT, FixedP: TailGroup lanes (G) per cell, which run its
+//    line-search trials side by side (pte_iterate_group), when those cells
+//    fit the grid at once (else one lane each).
+// Every cell runs to the end of its Newton loop.
 // Waves per SIMD (WPE): level 1, which resumes many parked cells at once,
 // runs two for 2..4 materials (latency hidden by the other wave) and one
-// for 5 and 6; level 2, a few cells that wander for hundreds of iterations
-// with nothing else on the GPU to hide their latency, always runs one, so
-// that its state stays in registers (the register file's second half takes
-// what the first cannot hold) instead of going to scratch memory.
+// for 5 and 6; level 2, a few cells with nothing else on the GPU to hide
+// their latency, always runs one, so that its state stays in registers (the
+// register file's second half takes what the first cannot hold) instead of
+// going to scratch memory.
 #ifndef SGM_TAIL_WAVES_PER_EU_LARGE
 #define SGM_TAIL_WAVES_PER_EU_LARGE 1
 #endif
@@ -349,81 +351,144 @@ template <int NM>
 constexpr int TailWavesL1() {
   return NM >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2;
 }
-template <int NM, int METHOD, int WPE>
+
+// Level 2 of a RhoT tail: the block's one wave takes record items (wave
+// slot, position) from the bin's queue pk.l2_next until none is left, and
+// runs each remaining cell with all 64 lanes (pte_iterate_wide). The
+// fixed-point probe's copy sits in LDS.
+// The loop keeps the wave converged: every lane executes every statement
+// (the queue's atomic adds 1 from lane 0 and 0 from the others; all lanes
+// write the same probe words and the same results), and every value that
+// steers the loop (the item, whether the cell is done, whether to probe or
+// skip) goes through readfirstlane, a scalar the compiler knows to be
+// wave-uniform. A lane-0-only branch, or a value broadcast with a lane
+// shuffle, would count as divergent, and the compiler may then split the
+// loop so that some lanes run an iteration without the others, which the
+// cross-lane gathers of each pass cannot allow.
+template <int NM>
+__device__ __forceinline__ void tail_wide(const PteArgs &a, const MixParamsDev &P,
+                                          const ParkArgs &pk, int lane) {
+  using Sys = SysOf<NM, PTE_RHOT, false>;
+  constexpr int per_slot = 64 / TailGroup(PTE_RHOT, NM);
+  constexpr int PW = ParkWordCount<Sys>();
+  __shared__ double probe_lds[PW];
+  const long items = (long)pk.nwaves * per_slot;
+  // these waves carry the bin's longest dependency chains: let them issue
+  // ahead of other kernels' waves on their CU
+  __builtin_amdgcn_s_setprio(3);
+  for (;;) {
+    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(pk.l2_next, lane == 0 ? 1 : 0));
+    if (item >= items) break;
+    const int w = item / per_slot;
+    const int r = pk.wave_run[w];
+    const int nr = min(pk.run_count[r], pk.run_cap[r]);
+    const int i = pk.wave_first[w] + item % per_slot;
+    if (i >= nr) continue;
+    const long slot = (long)pk.run_base[r] + i;
+    const long c = pk.cell_of[slot];
+    if (c < 0) continue;  // finished in level 1
+    Sys s;
+    DriverState d;
+    CellStatus st;
+    ParkLoad(s, d, st, pk.rec + slot * pk.stride);
+    s.P = P;
+    fixed_point_probe(s, d, st, probe_lds);  // the copy of the loaded state
+    ProbeGate gate;
+    gate.start(d.err);
+    for (;;) {
+      const bool done = pte_iterate_wide<NM>(s, d, st, lane);
+      if (__builtin_amdgcn_readfirstlane(done ? 1 : 0)) break;
+      const bool due = gate.due(d.err);
+      if (__builtin_amdgcn_readfirstlane(due ? 1 : 0)) {
+        const bool same = fixed_point_probe(s, d, st, probe_lds);
+        if (gate.skip(__builtin_amdgcn_readfirstlane(same ? 1 : 0) != 0)) skip_to_last_pass(d);
+      }
+    }
+    pte_end<NM>(s, d, st);
+    store_cell<NM>(s, st, a, c, a.offsets[c]);
+  }
+}
+
+template <int NM, int METHOD, int WPE, int LEVEL>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixParamsDev P,
-                                                                ParkArgs pk, int level,
-                                                                int resident) {
+                                                                ParkArgs pk, int resident) {
   constexpr int G = TailGroup(METHOD, NM);
   constexpr int per_slot = 64 / G;  // records per wave slot
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
   const int lane = (int)threadIdx.x;
   const bool two_levels = G > 1 && pk.after2 > 0 && pk.l2_count && P.line_search_max_iter <= G;
-  // level 2: groups while its cells fit the GPU's resident waves at once;
-  // past that the level is throughput-bound and keeps one lane per cell
-  bool grp = false;
-  if (level == 2) {
-    if (!two_levels) return;
-    grp = (long)(*pk.l2_count) * G <= 64L * (long)resident;
-  }
-  const int lpc = grp ? G : 1;  // lanes per cell
-  const int spb = grp ? 1 : G;  // wave slots per block step
-  const int g = lane % lpc, base = lane - g;
-  for (long W = blockIdx.x; W * spb < pk.nwaves; W += gridDim.x) {
-    const long w = grp ? W : W * G + lane / per_slot;
-    if (w >= pk.nwaves) continue;
-    const int r = pk.wave_run[w];
-    const int nr = min(pk.run_count[r], pk.run_cap[r]);
-    const int i = pk.wave_first[w] + (grp ? lane / G : lane % per_slot);
-    if (i >= nr) continue;  // in a group, the whole group skips
-    const long slot = (long)pk.run_base[r] + i;
-    const long c = pk.cell_of[slot];
-    if (c < 0) continue;  // finished in level 1
-    const int off = a.offsets[c];
-    SysOf<NM, METHOD, false> s;
-    DriverState d;
-    CellStatus st;
-    // the record, free once loaded, keeps the state before each pass for the
-    // fixed-point probe (in a group, written by its first lane only)
-    double *rec = pk.rec + slot * pk.stride;
-    ParkLoad(s, d, st, rec);
-    s.P = P;
-    bool moved = false;
-    if constexpr (G > 1) {
-      if (grp) {
-        // the group's probe copy in LDS: a pass's comparison and store then
-        // cost LDS latency instead of a round trip to memory
-        constexpr int PW = ParkWordCount<SysOf<NM, METHOD, false>>();
-        __shared__ double probe_lds[64 / G][PW];
-        double *pw = probe_lds[lane / G];
-        if (g == 0)  // the one lane that probes
-          for (int k = 0; k < PW; ++k) pw[k] = rec[k];
-        while (!pte_iterate_group<NM, G>(s, d, st, g, base)) {
-          int same = 0;
-          if (g == 0) same = fixed_point_probe(s, d, st, pw) ? 1 : 0;
-          if (__shfl(same, base, 64)) skip_to_last_pass(d);
+  if constexpr (LEVEL == 2 && METHOD == PTE_RHOT) {
+    if (two_levels) tail_wide<NM>(a, P, pk, lane);
+  } else {
+    // level 2: groups while its cells fit the GPU's resident waves at once;
+    // past that the level is throughput-bound and keeps one lane per cell
+    bool grp = false;
+    if constexpr (LEVEL == 2) {
+      if (!two_levels) return;
+      grp = (long)(*pk.l2_count) * G <= 64L * (long)resident;
+    }
+    const int lpc = grp ? G : 1;  // lanes per cell
+    const int spb = grp ? 1 : G;  // wave slots per block step
+    const int g = lane % lpc, base = lane - g;
+    for (long W = blockIdx.x; W * spb < pk.nwaves; W += gridDim.x) {
+      const long w = grp ? W : W * G + lane / per_slot;
+      if (w >= pk.nwaves) continue;
+      const int r = pk.wave_run[w];
+      const int nr = min(pk.run_count[r], pk.run_cap[r]);
+      const int i = pk.wave_first[w] + (grp ? lane / G : lane % per_slot);
+      if (i >= nr) continue;  // in a group, the whole group skips
+      const long slot = (long)pk.run_base[r] + i;
+      const long c = pk.cell_of[slot];
+      if (c < 0) continue;  // finished in level 1
+      const int off = a.offsets[c];
+      SysOf<NM, METHOD, false> s;
+      DriverState d;
+      CellStatus st;
+      // the record, free once loaded, keeps the probe's copy of the state
+      // when one lane runs the cell (a group keeps its copy in LDS)
+      double *rec = pk.rec + slot * pk.stride;
+      ParkLoad(s, d, st, rec);
+      s.P = P;
+      ProbeGate gate;
+      gate.start(d.err);
+      bool moved = false;
+      if constexpr (G > 1 && LEVEL == 2) {
+        if (grp) {
+          // the group's probe copy in LDS: a probe's comparison and store
+          // then cost LDS latency instead of a round trip to memory. Every
+          // lane of the group probes (the same words, the same result), so
+          // no lane branches on its position in the group
+          constexpr int PW = ParkWordCount<SysOf<NM, METHOD, false>>();
+          __shared__ double probe_lds[64 / G][PW];
+          double *pw = probe_lds[lane / G];
+          fixed_point_probe(s, d, st, pw);  // the copy of the loaded state
+          while (!pte_iterate_group<NM, G>(s, d, st, g, base))
+            if (gate.due(d.err) && gate.skip(fixed_point_probe(s, d, st, pw)))
+              skip_to_last_pass(d);
         }
       }
-    }
-    if (!grp) {
-      const bool can_move = level == 1 && two_levels;
-      const int stop = d.niter + pk.after2;
-      while (!pte_iterate<NM>(s, d, st)) {
-        if (fixed_point_probe(s, d, st, rec)) skip_to_last_pass(d);
-        if (can_move && d.niter >= stop) {
-          ParkSave(s, d, st, rec);
-          atomicAdd(pk.l2_count, 1);
-          moved = true;
-          break;
+      if (!grp) {
+        const bool can_move = LEVEL == 1 && two_levels;
+        const int stop = d.niter + pk.after2;
+        while (!pte_iterate<NM>(s, d, st)) {
+          if (gate.due(d.err) && gate.skip(fixed_point_probe(s, d, st, rec)))
+            skip_to_last_pass(d);
+          if (can_move && d.niter >= stop) {
+            ParkSave(s, d, st, rec);
+            atomicAdd(pk.l2_count, 1);
+            moved = true;
+            break;
+          }
         }
       }
-    }
-    if (moved) continue;
-    pte_end<NM>(s, d, st);
-    if (g == 0) {
-      store_cell<NM>(s, st, a, c, off);
-      if (level == 1 && two_levels) pk.cell_of[slot] = -1;
+      if (moved) continue;
+      pte_end<NM>(s, d, st);
+      if (g == 0) {
+        store_cell<NM>(s, st, a, c, off);
+        if (LEVEL == 1 && two_levels) pk.cell_of[slot] = -1;
+      }
     }
   }
 }
@@ -476,21 +541,25 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   // occupancy (one wave per SIMD from 5 materials, else two)
   const int ncu = device_cu_count();
   const int cap = (nm >= 5 ? SGM_TAIL_WAVES_PER_EU_LARGE : 2) * 4 * ncu;
-  const int g1 = (pk.nwaves + TailGroup(METHOD, nm) - 1) / TailGroup(METHOD, nm);
+  const int G = TailGroup(METHOD, nm);
+  const int g1 = (pk.nwaves + G - 1) / G;
   const int g = g1 < cap ? g1 : cap;
-  const bool l2 = TailGroup(METHOD, nm) > 1 && pk.after2 > 0 && pk.l2_count &&
-                  P.line_search_max_iter <= TailGroup(METHOD, nm);
-  // level 2: one block per wave slot, so that no block walks two slots that
-  // both hold long-running cells (its few cells would then run one after the
-  // other); empty slots exit at once
+  const bool l2 = G > 1 && pk.after2 > 0 && pk.l2_count &&
+                  P.line_search_max_iter <= G && (METHOD != PTE_RHOT || pk.l2_next);
+  // level 2, RhoT: one wave per CU (a few hundred cells at most remain, and
+  // no two of this kernel's long chains share a CU), each taking cells from
+  // the queue; FixedT, FixedP: one block per wave slot, so that no block
+  // walks two slots that both hold long-running cells (its few cells would
+  // then run one after the other); empty slots exit at once
+  const long items = (long)pk.nwaves * (64 / G);
+  const int g2 = METHOD == PTE_RHOT ? (int)(items < ncu ? items : ncu) : pk.nwaves;
   for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
-    const int gl = level == 1 ? g : pk.nwaves;
 #define SGM_TAIL(K)                                                                            \
   case K:                                                                                      \
     if (level == 1)                                                                            \
-      pte_tail_kernel<K, METHOD, TailWavesL1<K>()><<<gl, 64, lds, s>>>(a, P, pk, level, cap);  \
+      pte_tail_kernel<K, METHOD, TailWavesL1<K>(), 1><<<g, 64, lds, s>>>(a, P, pk, cap);       \
     else                                                                                       \
-      pte_tail_kernel<K, METHOD, 1><<<gl, 64, lds, s>>>(a, P, pk, level, 4 * ncu);             \
+      pte_tail_kernel<K, METHOD, 1, 2><<<g2, 64, lds, s>>>(a, P, pk, 4 * ncu);                 \
     break;
     switch (nm) {
       SGM_TAIL(2)
