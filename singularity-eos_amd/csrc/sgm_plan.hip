@@ -217,12 +217,17 @@ hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixPara
 }
 
 // Order of the bins on the launch stream: the bins whose tail kernels run
-// longest first (5, 4, 6, 3 and 2 materials on the uneven workload: 17, 17,
-// 15, 11 and 4 ms), so their tails overlap the later bins' persistent
-// kernels; then the list bins.
+// longest (5, 4 and 6 materials on the uneven workload: 14, 13 and 12 ms
+// beside the other kernels) early, so their tails overlap the later bins'
+// persistent kernels; then the list bins. The runtime spreads a captured
+// graph's branches over the process's hardware queues (four by default),
+// each bin's persistent kernel on the next queue with its tail behind it,
+// so the fifth bin's persistent kernel waits behind the first bin's tail:
+// the first bin is 3 materials, whose tail (10 ms) ends by the time the
+// fifth (2 materials) is due.
 std::vector<size_t> bin_order(const sgm_plan *p) {
   std::vector<size_t> order;
-  for (int nm : {5, 4, 6, 3, 2})
+  for (int nm : {3, 5, 4, 6, 2})
     for (size_t i = 0; i < p->bins.size(); ++i)
       if (p->bins[i].nm == nm) order.push_back(i);
   for (size_t i = 0; i < p->bins.size(); ++i)
