@@ -31,6 +31,16 @@ METRIC = "PTE closure cells/s at 1e-10 tol + p99 request latency, served vs plai
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# Per material-count bin of the headline workload, one execution of the bin's
+# kernels (pte_bin_kernel + both pte_tail_kernel levels): HBM bytes moved
+# (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the counters' reading of a
+# copy kernel of known size) and FP64 operations issued (SQ_INSTS_VALU_
+# {ADD,MUL,TRANS}_F64 x 64 lanes + FMA_F64 x 128). Measured with the
+# commands in profiles/pte_bins_uneven_2p20.md on the current kernels; None
+# where not measured.
+UNIT_HBM_BYTES = {}
+UNIT_FP64_OPS = {}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -187,7 +197,14 @@ def main():
     conv = torch.zeros(n, dtype=torch.int32, device=dev)
     niter = torch.zeros(n, dtype=torch.int32, device=dev)
 
+    # the plan (sorting the cells into material-count bins, straggler
+    # records, graph capture on first execute) is built once per cell
+    # topology, outside the timed steps; its cost is reported beside them
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
     plan = sgm.Plan(ctx, n, nnz, off, mat, 0, stream=stream)
+    torch.cuda.synchronize()
+    plan_build_ms = (time.perf_counter() - tb) * 1e3
     if args.autotune:
         log("[bench] autotuning launch shapes")
         plan.autotune(init["rho"], init["vfrac"], init["temp"], init["press"], vt, st,
@@ -263,6 +280,10 @@ def main():
                             tail_kernel_ms=float(kms[1]), block=b["block"],
                             cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
     extras = {"bins": per_bin, "step_ms": step_stats, "converged_frac": conv_frac,
+              "plan_build_ms": plan_build_ms,
+              "plan_build_note": "sgm_plan_create for the 2^20-cell topology (bins, straggler "
+                                 "records), measured once before the timed steps and not part "
+                                 "of them; a step with a new topology pays it once",
               "mean_newton_iters": mean_iter,
               "max_newton_iters": int(niter.max().item())}
     if not args.no_reorder:
@@ -296,16 +317,21 @@ def main():
         i0 = {k: d0[k].clone() for k in fields}
         o0 = torch.tensor(c0["offsets"], device=dev)
         m0 = torch.tensor(c0["mat_ids"], device=dev)
-        for it in range(2):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(5):
-                for k in fields:
-                    d0[k].copy_(i0[k])
-                ctx.pte(0, 65536, 131072, o0, m0, d0["rho"], d0["vfrac"], d0["sie"], d0["temp"],
-                        d0["press"], d0["vfrac_tot"], d0["sie_tot"], params=params, stream=stream)
-            torch.cuda.synchronize()
-            extras["plain_pte2_65536_cells_per_s"] = 65536 * 5 / (time.perf_counter() - t0)
+        c0_rates = {}
+        for tol, prm in (("1e-10", params), ("default", sgm.mix_params())):
+            for it in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    for k in fields:
+                        d0[k].copy_(i0[k])
+                    ctx.pte(0, 65536, 131072, o0, m0, d0["rho"], d0["vfrac"], d0["sie"],
+                            d0["temp"], d0["press"], d0["vfrac_tot"], d0["sie_tot"], params=prm,
+                            stream=stream)
+                torch.cuda.synchronize()
+                c0_rates[tol] = 65536 * 5 / (time.perf_counter() - t0)
+        extras["plain_pte2_65536_cells_per_s"] = c0_rates["1e-10"]
+        extras["plain_pte2_65536_cells_per_s_by_tol"] = c0_rates
         # table lookups: 2^24 forward (P, e) and inverse T(rho, e)
         nq = 1 << 24
         rq, Tq = wl.table_queries(nq, table)
@@ -461,6 +487,48 @@ def main():
                          f"cells: {n1 / c1dt:.4g} cells/s",
                "single_core_value": n1 / c1dt}
         log(f"[bench] cpu baseline: {cpu['value']:.4g} cells/s on {threads} threads")
+        # the two-material closure (65,536 cells) and the table lookups on a
+        # 2^20-query sample, on the same host threads
+        c0 = wl.pte2(65536)
+        m0 = np.tile(np.array([0, 2], dtype=np.int32), 65536)
+        c0_cpu = {}
+        for tol, prm in (("1e-10", params), ("default", sgm.mix_params())):
+            def c0_run():
+                r = {f: c0[f].copy() for f in fields}
+                t0 = time.perf_counter()
+                O.pte_solve(0, olist, c0["offsets"], m0, r["rho"], r["vfrac"], r["sie"],
+                            r["temp"], r["press"], c0["vfrac_tot"], c0["sie_tot"],
+                            prm.as_oracle_array(), nthreads=threads)
+                return time.perf_counter() - t0
+            c0_run()
+            c0_cpu[tol] = 65536 / float(np.median([c0_run() for _ in range(3)]))
+        nq1 = 1 << 20
+        rq1, Tq1 = wl.table_queries(nq1, table)
+        te = otab.eos()
+        F_P_RT, F_E_RT, F_T_RE = 3, 2, 1
+
+        def c1_run():
+            t0 = time.perf_counter()
+            te.vec(F_P_RT, rq1, Tq1, nthreads=threads)
+            eq1 = te.vec(F_E_RT, rq1, Tq1, nthreads=threads)
+            t1 = time.perf_counter()
+            te.vec(F_T_RE, rq1, eq1, nthreads=threads)
+            return t1 - t0, time.perf_counter() - t1
+        c1_run()
+        c1 = [c1_run() for _ in range(3)]
+        c1_fwd = nq1 / float(np.median([x[0] for x in c1]))
+        c1_inv = nq1 / float(np.median([x[1] for x in c1]))
+        extras["cpu_baselines_other_configs"] = {
+            "pte2_65536_cells_per_s": c0_cpu,
+            "table_forward_queries_per_s": c1_fwd,
+            "table_inverse_queries_per_s": c1_inv,
+            "cores": threads, "kind": "port",
+            "sample": "two-material closure: all 65,536 cells (IdealGas + StiffGas), median of "
+                      "3; table: 2^20 of the 2^24 log-uniform queries on the 256x512 "
+                      "Gruneisen-sampled table, P(rho,T) and e(rho,T) forward, T(rho,e) "
+                      "inverse, median of 3"}
+        log(f"[bench] cpu two-material closure: {c0_cpu}; table forward {c1_fwd:.4g}, "
+            f"inverse {c1_inv:.4g} queries/s")
         fl = O.count_pte_flops(0, specs, oo, me, a["rho"][:k], a["vfrac"][:k], a["temp"][:k],
                                a["press"][:k], a["vfrac_tot"][:ns], a["sie_tot"][:ns],
                                params.as_oracle_array(), table=otab)
@@ -505,17 +573,25 @@ def main():
     gbs = alg_bytes / (kern_ms * 1e-3) / 1e9
     kname = f"pte_bin_kernel<{dom['nmat']}, 0> + pte_tail_kernel<{dom['nmat']}, 0> (RhoT)"
     split = {"bulk_kernel_ms": dom["bulk_kernel_ms"], "tail_kernel_ms": dom["tail_kernel_ms"]}
+    traffic = UNIT_HBM_BYTES.get(nm)
     hbm = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+           "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
            "bytes_per_cell": bytes_per_cell(nm)}
     if flops_per_cell and nm in flops_per_cell:
         tf = dom["cells"] * flops_per_cell[nm] / (kern_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": tf / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kname,
-                    "cells": dom["cells"], "ms": kern_ms, **split,
+        roofline = {"bound": "fp64_valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS, "traffic": traffic,
+                    "kernel": kname, "cells": dom["cells"], "ms": kern_ms, **split,
                     "flops_per_cell": flops_per_cell[nm],
-                    "pipe": "FP64 on the vector ALUs; 78.6 TFLOP/s is MI355X's FP64 peak "
-                            "for vector and matrix alike (no MFMA in this kernel)"}
+                    "algorithmic_bytes": alg_bytes,
+                    "pipe": "FP64 on the vector ALUs (no MFMA in this kernel); 78.6 TFLOP/s "
+                            "is MI355X's FP64 peak; achieved = the oracle's counted operations "
+                            "per cell (every +,-,*,/ and math call once) x cells / the bin's "
+                            "bulk + tail kernel time"}
+        ops = UNIT_FP64_OPS.get(nm)
+        if ops:
+            roofline["rocprof_fp64_TFLOPs"] = ops / (kern_ms * 1e-3) / 1e12
+            roofline["rocprof_fp64_frac"] = roofline["rocprof_fp64_TFLOPs"] / FP64_PEAK_TFLOPS
         extras["roofline_hbm"] = hbm
         extras["counted_flops_per_cell_by_nmat"] = flops_per_cell
     else:

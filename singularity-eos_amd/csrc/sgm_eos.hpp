@@ -173,7 +173,18 @@ struct InvT {
   int where, iters;
   Lam lam;
 };
-SGM_NOINL InvT lT_from_lRho_sie(const TableDev &t, Real lRho, Real sie, Lam lam) {
+// The table inversions (regula falsi over a grid column) run out of line in
+// the closure kernels, where an inlined copy at every material's P(rho, e)
+// call site would grow code that is already instruction-cache bound, and
+// inline in the vector kernels (SGM_TABLE_INVERSE_INLINE), where an
+// out-of-line call would save the callee's registers to scratch memory at
+// every point. Same arithmetic either way.
+#ifdef SGM_TABLE_INVERSE_INLINE
+#define SGM_TABLE_INV SGM_HD
+#else
+#define SGM_TABLE_INV SGM_NOINL
+#endif
+SGM_TABLE_INV InvT lT_from_lRho_sie(const TableDev &t, Real lRho, Real sie, Lam lam) {
   InvT o;
   Real lT;
   const int where = loc_rho_sie(t, lRho, sie);
@@ -208,7 +219,7 @@ struct InvRho {
   int where;
   Lam lam;
 };
-SGM_NOINL InvRho lRho_from_P_lT(const TableDev &t, Real P, Real lT, Lam lam) {
+SGM_TABLE_INV InvRho lRho_from_P_lT(const TableDev &t, Real P, Real lT, Lam lam) {
   const Real rhopmin = lRho_(t, lerp_T(t, t.rho_at_pmin, lT));
   Real lRho;
   int where;

@@ -6,6 +6,8 @@
 // is staged once per block in LDS, inputs stream as structure-of-arrays with
 // 16-byte vector loads where alignment allows (two points per lane), and the
 // grid is sized to keep every CU of the 8 XCDs busy with a grid-stride loop.
+// the table inversions inline here (see sgm_eos.hpp)
+#define SGM_TABLE_INVERSE_INLINE
 #include "sgm_internal.hpp"
 
 namespace sgm {
@@ -13,9 +15,8 @@ namespace sgm {
 namespace {
 
 template <int FUNC, int KIND>
-__device__ __forceinline__ Real vec_point(const EosDev &e, Real a, Real b, Lam &lam) {
-  EosDev ek = e;
-  ek.kind = KIND; // lets the compiler fold the dispatch switch
+__device__ __forceinline__ Real vec_point(const EosDev &ek, Real a, Real b, Lam &lam) {
+  __builtin_assume(ek.kind == KIND);  // lets the compiler fold the dispatch switch
   switch (FUNC) {
   case SGM_F_P_RHO_SIE: return eos_P_re(ek, a, b, lam);
   case SGM_F_T_RHO_SIE: return eos_T_re(ek, a, b, lam);
@@ -102,8 +103,8 @@ __global__ void __launch_bounds__(256)
                 int lstride, double *lambda, int *errs) {
   __shared__ EosDev es;
   stage_eos(es, reg + mat);
-  EosDev ek = es;
-  ek.kind = KIND;
+  const EosDev &ek = es;  // read in place (a private copy would sit in scratch memory)
+  __builtin_assume(ek.kind == KIND);
   // a modified material's FillEos may leave a requested output unwritten (a
   // ScaledEOS over a (rho, sie) model fills a copy of the energy,
   // scaled_eos.hpp:155-160): its arrays are read in full so that such an
@@ -154,8 +155,8 @@ __global__ void __launch_bounds__(256)
                    double *rho, double *sie, long n, int lstride, double *lambda, int *errs) {
   __shared__ EosDev es;
   stage_eos(es, reg + mat);
-  EosDev ek = es;
-  ek.kind = KIND;
+  const EosDev &ek = es;
+  __builtin_assume(ek.kind == KIND);
   const long nthr = (long)gridDim.x * blockDim.x;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nthr) {
     Lam lam;
