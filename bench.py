@@ -38,8 +38,8 @@ FP64_PEAK_TFLOPS = 78.6
 # {ADD,MUL,TRANS}_F64 x 64 lanes + FMA_F64 x 128). Measured with the
 # commands in profiles/pte_bins_uneven_2p20.md on the current kernels; None
 # where not measured.
-UNIT_HBM_BYTES = {}
-UNIT_FP64_OPS = {}
+UNIT_HBM_BYTES = {2: 5.736e8, 3: 7.072e8, 4: 1.3336e9, 5: 2.3145e9, 6: 2.486e9}
+UNIT_FP64_OPS = {2: 1.137e10, 3: 2.156e10, 4: 1.969e10, 5: 1.984e10, 6: 1.655e10}
 
 
 def log(*a):
@@ -278,7 +278,8 @@ def main():
         # kernel_ms: the persistent kernel and its straggler tail kernel alone
         per_bin.append(dict(nmat=nm, cells=cells, ms=ms, bulk_kernel_ms=float(kms[0]),
                             tail_kernel_ms=float(kms[1]), block=b["block"],
-                            cells_per_wave=b["cells_per_wave"], grid=b["grid"]))
+                            cells_per_wave=b["cells_per_wave"], grid=b["grid"],
+                            tail_level2_cells=plan.tail_counts(stream=stream)[bi]))
     extras = {"bins": per_bin, "step_ms": step_stats, "converged_frac": conv_frac,
               "plan_build_ms": plan_build_ms,
               "plan_build_note": "sgm_plan_create for the 2^20-cell topology (bins, straggler "

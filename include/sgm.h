@@ -372,6 +372,11 @@ int sgm_plan_execute_bin(sgm_plan *plan, int bin, double *rho, double *vfrac, do
 int sgm_plan_reorder(sgm_plan *plan, const int *cost, sgm_stream_t stream);
 /* host: per-bin description, out[6*i..] = {nmat, ncells, block, cells_per_wave, grid, start} */
 int sgm_plan_describe(const sgm_plan *plan, long *out, int max_bins);
+/* No reference counterpart (diagnostics of the straggler tails). After an
+ * execution on `stream`: out[i] = the number of bin i's cells that its tail
+ * resumed in the second level (still iterating after the first level), 0 for
+ * bins without parking. Synchronizes the stream; returns the bin count. */
+int sgm_plan_tail_counts(const sgm_plan *plan, int *out, int max_bins, sgm_stream_t stream);
 int sgm_plan_destroy(sgm_plan *plan);
 
 

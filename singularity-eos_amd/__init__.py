@@ -134,6 +134,7 @@ _SIGS = {
     "sgm_plan_execute_bin": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                   C.POINTER(MixParams), PteStatus, vp, vp]),
     "sgm_plan_describe": (ci, [vp, C.POINTER(C.c_long), ci]),
+    "sgm_plan_tail_counts": (ci, [vp, C.POINTER(C.c_int), ci, vp]),
     "sgm_plan_reorder": (ci, [vp, vp, vp]),
     "sgm_plan_destroy": (ci, [vp]),
     "sgm_mix_params_default": (None, [C.POINTER(MixParams)]),
@@ -412,6 +413,14 @@ class Plan:
         return [dict(nmat=buf[6 * i], ncells=buf[6 * i + 1], block=buf[6 * i + 2],
                      cells_per_wave=buf[6 * i + 3], grid=buf[6 * i + 4], start=buf[6 * i + 5])
                 for i in range(min(n, 16))]
+
+    def tail_counts(self, stream=0):
+        """Per bin: cells the last execution's tail resumed in its second level."""
+        buf = (C.c_int * 16)()
+        n = lib().sgm_plan_tail_counts(self.h, buf, 16, stream)
+        if n < 0:
+            check(n, "plan_tail_counts")
+        return [buf[i] for i in range(min(n, 16))]
 
     def close(self):
         if self.h:

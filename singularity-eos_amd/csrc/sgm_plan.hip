@@ -477,6 +477,21 @@ int sgm_plan_describe(const sgm_plan *p, long *out, int max_bins) {
   return (int)p->bins.size();
 }
 
+int sgm_plan_tail_counts(const sgm_plan *p, int *out, int max_bins, sgm_stream_t stream) {
+  if (!p || !out || max_bins < 0) return SGM_E_ARG;
+  const int nb = (int)p->bins.size();
+  std::vector<int> c(2 * (size_t)std::max(1, nb), 0);
+  if (p->d_l2_counts) {
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(c.data(), p->d_l2_counts, sizeof(int) * c.size(), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return SGM_E_HIP;
+  }
+  for (int i = 0; i < nb && i < max_bins; ++i) out[i] = p->bins[i].park.rec ? c[2 * i] : 0;
+  return nb;
+}
+
 // Puts, within every material count, the cells whose cost (typically the
 // previous execution's Newton iteration counts) reaches 4x the mean cost
 // first: the slowest cells then start with the bulk instead of trailing it.

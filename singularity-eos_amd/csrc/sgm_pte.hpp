@@ -1612,6 +1612,7 @@ struct DriverState {
 
 template <class F>
 SGM_HD constexpr void VisitDriver(DriverState &d, CellStatus &st, F &f) {
+  // (fixed_point_probe skips words 1 and 9 of this order: the counters)
   f(d.err);
   f(d.niter);
   f(d.small_step_iters);
@@ -1661,6 +1662,10 @@ template <int NM, class S>
 SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &st) {
   st.converged = 0;
   st.niter = 0;
+  // set by pte_end; defined here so that a parked state (and the fixed-point
+  // probe's comparison of it) never holds an indeterminate word
+  st.err = 0;
+  st.residual = 0;
   d.err = s.Init(Tin);
   d.converged = false;
   d.close_enough = false;
@@ -1757,33 +1762,42 @@ __device__ __forceinline__ void group_take(S &s, DriverState &d, CellStatus &st,
 // further pass of the loop body then repeats that state until the iteration
 // cap, where the last pass also applies the cap's slow-convergence test.
 // fixed_point_probe compares the cell's state (every word Visit and
-// VisitDriver see, the iteration counters aside) with the copy in w, then
-// stores the current state there; true when the two are identical. A loop
-// that finds its state unchanged by one pass may skip to the last pass: the
-// skipped passes would reproduce the same state, so the state, counters and
-// status after the last pass are those of the full loop.
+// VisitDriver see, the two iteration counters aside) with the copy in w,
+// then stores the current state there (the layout of ParkSave, so a park
+// record is a valid copy of the state it holds); true when the two are
+// identical. A loop that finds its state unchanged by one pass may skip to
+// the last pass: the skipped passes would reproduce the same state, so the
+// state, counters and status after the last pass are those of the full loop.
 struct FixedPointProbe {
   double *w;
   int k;
   bool same;
+  int skip0, skip1;  // word positions not compared (the iteration counters)
   template <class T>
   SGM_HD void operator()(T &x) {
     const double v = (double)x;
-    same = same && __builtin_bit_cast(unsigned long long, v) ==
-                       __builtin_bit_cast(unsigned long long, w[k]);
+    same = same && (k == skip0 || k == skip1 ||
+                    __builtin_bit_cast(unsigned long long, v) ==
+                        __builtin_bit_cast(unsigned long long, w[k]));
     w[k++] = v;
   }
 };
+// the number of words Visit writes for a formulation's solver state
+template <class Sys>
+constexpr int SolverWordCount() {
+  Sys s{};
+  int k = 0;
+  auto count = [&](auto &) { ++k; };
+  s.Visit(count);
+  return k;
+}
 template <class Sys>
 SGM_HD bool fixed_point_probe(Sys &s, DriverState &d, CellStatus &st, double *w) {
-  const int n0 = d.niter, s0 = st.niter;
-  d.niter = 0;
-  st.niter = 0;
-  FixedPointProbe f{w, 0, true};
+  // VisitDriver's order: d.err, d.niter, ..., st.converged, st.niter, ...
+  constexpr int base = SolverWordCount<Sys>();
+  FixedPointProbe f{w, 0, true, base + 1, base + 9};
   s.Visit(f);
   VisitDriver(d, st, f);
-  d.niter = n0;
-  st.niter = s0;
   return f.same;
 }
 // after a pass whose state matched the probe copy: resume at the last pass

@@ -324,26 +324,21 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
   }
 }
 
-// Resumes the parked cells of one bin in two levels (LEVEL). Level 1: a
+// Resumes the parked cells of one bin in two levels. Level 1 (LEVEL 1): a
 // persistent grid of one-wave blocks walks the bin's tail wave slots (each
 // on records of a single signature run), one lane per cell; a cell that
 // reaches a fixed point of the Newton loop skips to its last pass, and a
 // cell still iterating pk.after2 iterations later is saved back into its
 // record for level 2 (finished cells clear their record's cell id). Level 2
-// finishes the few cells that wander for hundreds of iterations, where each
-// iteration's latency is what counts:
-//  * RhoT: one wave per cell (tail_wide), cells taken in record order from
-//    a queue, so a wave that finishes a short cell takes the next one;
-//  * FixedT, FixedP: TailGroup lanes (G) per cell, which run its
-//    line-search trials side by side (pte_iterate_group), when those cells
-//    fit the grid at once (else one lane each).
-// Every cell runs to the end of its Newton loop.
-// Waves per SIMD (WPE): level 1, which resumes many parked cells at once,
-// runs two for 2..4 materials (latency hidden by the other wave) and one
-// for 5 and 6; level 2, a few cells with nothing else on the GPU to hide
-// their latency, always runs one, so that its state stays in registers (the
-// register file's second half takes what the first cannot hold) instead of
-// going to scratch memory.
+// finishes the cells that keep wandering. When few remain (RhoT, at most
+// wide_max cells: a latency problem) each runs on a whole wave (LEVEL 3,
+// tail_wide), cells taken in record order from a queue. Otherwise (LEVEL 2:
+// every FixedT / FixedP level 2, and RhoT with more cells, as after cold
+// starts: a throughput problem) TailGroup lanes (G) share each cell and run
+// its line-search trials side by side (pte_iterate_group) while those cells
+// fit the grid at once, else one lane each. RhoT launches both kernels, each
+// of which reads the level-2 count and returns at once when the other
+// applies. Every cell runs to the end of its Newton loop.
 #ifndef SGM_TAIL_WAVES_PER_EU_LARGE
 #define SGM_TAIL_WAVES_PER_EU_LARGE 1
 #endif
@@ -412,22 +407,26 @@ __device__ __forceinline__ void tail_wide(const PteArgs &a, const MixParamsDev &
 template <int NM, int METHOD, int WPE, int LEVEL>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixParamsDev P,
-                                                                ParkArgs pk, int resident) {
+                                                                ParkArgs pk, int resident,
+                                                                int wide_max) {
   constexpr int G = TailGroup(METHOD, NM);
   constexpr int per_slot = 64 / G;  // records per wave slot
   extern __shared__ EosDev lds[];
   stage_registry(lds, a.d_eos, a.nreg);
   const int lane = (int)threadIdx.x;
   const bool two_levels = G > 1 && pk.after2 > 0 && pk.l2_count && P.line_search_max_iter <= G;
-  if constexpr (LEVEL == 2 && METHOD == PTE_RHOT) {
-    if (two_levels) tail_wide<NM>(a, P, pk, lane);
+  if constexpr (LEVEL == 3) {
+    static_assert(METHOD == PTE_RHOT, "whole-wave cells: RhoT");
+    if (two_levels && *pk.l2_count <= wide_max) tail_wide<NM>(a, P, pk, lane);
   } else {
     // level 2: groups while its cells fit the GPU's resident waves at once;
     // past that the level is throughput-bound and keeps one lane per cell
     bool grp = false;
     if constexpr (LEVEL == 2) {
       if (!two_levels) return;
-      grp = (long)(*pk.l2_count) * G <= 64L * (long)resident;
+      const int l2 = *pk.l2_count;
+      if (METHOD == PTE_RHOT && l2 <= wide_max) return;  // the whole-wave kernel's
+      grp = (long)l2 * G <= 64L * (long)resident;
     }
     const int lpc = grp ? G : 1;  // lanes per cell
     const int spb = grp ? 1 : G;  // wave slots per block step
@@ -473,8 +472,10 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixPar
         const bool can_move = LEVEL == 1 && two_levels;
         const int stop = d.niter + pk.after2;
         while (!pte_iterate<NM>(s, d, st)) {
-          if (gate.due(d.err) && gate.skip(fixed_point_probe(s, d, st, rec)))
+          if (gate.due(d.err) && gate.skip(fixed_point_probe(s, d, st, rec))) {
             skip_to_last_pass(d);
+            continue;  // its last pass runs here, not in level 2
+          }
           if (can_move && d.niter >= stop) {
             ParkSave(s, d, st, rec);
             atomicAdd(pk.l2_count, 1);
@@ -546,20 +547,28 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   const int g = g1 < cap ? g1 : cap;
   const bool l2 = G > 1 && pk.after2 > 0 && pk.l2_count &&
                   P.line_search_max_iter <= G && (METHOD != PTE_RHOT || pk.l2_next);
-  // level 2, RhoT: one wave per CU (a few hundred cells at most remain, and
-  // no two of this kernel's long chains share a CU), each taking cells from
-  // the queue; FixedT, FixedP: one block per wave slot, so that no block
-  // walks two slots that both hold long-running cells (its few cells would
-  // then run one after the other); empty slots exit at once
+  // level 2 on whole waves (RhoT): one wave per CU, so no two of the
+  // kernel's long chains share a CU, each taking cells from the queue; in
+  // groups: one block per wave slot, so that no block walks two slots that
+  // both hold long-running cells (its few cells would then run one after
+  // the other); empty slots exit at once
   const long items = (long)pk.nwaves * (64 / G);
-  const int g2 = METHOD == PTE_RHOT ? (int)(items < ncu ? items : ncu) : pk.nwaves;
+  const int g3 = (int)(items < ncu ? items : ncu);
+  // whole-wave level 2 while at most four cells per CU remain (the CU's
+  // wave takes them one after another from the queue)
+  const int wide_max = 4 * ncu;
   for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
 #define SGM_TAIL(K)                                                                            \
   case K:                                                                                      \
-    if (level == 1)                                                                            \
-      pte_tail_kernel<K, METHOD, TailWavesL1<K>(), 1><<<g, 64, lds, s>>>(a, P, pk, cap);       \
-    else                                                                                       \
-      pte_tail_kernel<K, METHOD, 1, 2><<<g2, 64, lds, s>>>(a, P, pk, 4 * ncu);                 \
+    if (level == 1) {                                                                          \
+      pte_tail_kernel<K, METHOD, TailWavesL1<K>(), 1><<<g, 64, lds, s>>>(a, P, pk, cap,        \
+                                                                         wide_max);           \
+    } else {                                                                                   \
+      if constexpr (METHOD == PTE_RHOT)                                                        \
+        pte_tail_kernel<K, METHOD, 1, 3><<<g3, 64, lds, s>>>(a, P, pk, 4 * ncu, wide_max);     \
+      pte_tail_kernel<K, METHOD, 1, 2><<<pk.nwaves, 64, lds, s>>>(a, P, pk, 4 * ncu,           \
+                                                                  wide_max);                   \
+    }                                                                                          \
     break;
     switch (nm) {
       SGM_TAIL(2)
