@@ -79,6 +79,10 @@ enum PteMethod {
 constexpr int TailGroup(int method, int nm) {
   return (method != PTE_PT && nm >= SGM_TAIL_GROUP_MIN_NM) ? 8 : 1;
 }
+// Whether a bin's tail may run its second level on whole waves (RhoT from 4
+// materials: one material per lane pays for the wave's gathers only where
+// there are enough materials to spread)
+constexpr bool TailWide(int method, int nm) { return method == PTE_RHOT && nm >= 4; }
 
 // Per-method launchers, one translation unit pair per method
 // (sgm_pte_plain_<name>.hip: plain + list kernels; sgm_pte_bins_<name>.hip:

@@ -330,15 +330,16 @@ __attribute__((amdgpu_waves_per_eu(SGM_BIN_WAVES_PER_EU,
 // reaches a fixed point of the Newton loop skips to its last pass, and a
 // cell still iterating pk.after2 iterations later is saved back into its
 // record for level 2 (finished cells clear their record's cell id). Level 2
-// finishes the cells that keep wandering. When few remain (RhoT, at most
-// wide_max cells: a latency problem) each runs on a whole wave (LEVEL 3,
-// tail_wide), cells taken in record order from a queue. Otherwise (LEVEL 2:
-// every FixedT / FixedP level 2, and RhoT with more cells, as after cold
-// starts: a throughput problem) TailGroup lanes (G) share each cell and run
-// its line-search trials side by side (pte_iterate_group) while those cells
-// fit the grid at once, else one lane each. RhoT launches both kernels, each
-// of which reads the level-2 count and returns at once when the other
-// applies. Every cell runs to the end of its Newton loop.
+// finishes the cells that keep wandering. When few remain (RhoT with 4+
+// materials, TailWide, at most wide_max cells: a latency problem) each runs
+// on a whole wave (LEVEL 3, tail_wide), cells taken in record order from a
+// queue. Otherwise (LEVEL 2: FixedT, FixedP, RhoT with fewer materials, and
+// RhoT with more cells, as after cold starts: a throughput problem)
+// TailGroup lanes (G) share each cell and run its line-search trials side by
+// side (pte_iterate_group) while those cells fit the grid at once, else one
+// lane each. TailWide bins launch both kernels, each of which reads the
+// level-2 count and returns at once when the other applies. Every cell runs
+// to the end of its Newton loop.
 #ifndef SGM_TAIL_WAVES_PER_EU_LARGE
 #define SGM_TAIL_WAVES_PER_EU_LARGE 1
 #endif
@@ -416,7 +417,7 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixPar
   const int lane = (int)threadIdx.x;
   const bool two_levels = G > 1 && pk.after2 > 0 && pk.l2_count && P.line_search_max_iter <= G;
   if constexpr (LEVEL == 3) {
-    static_assert(METHOD == PTE_RHOT, "whole-wave cells: RhoT");
+    static_assert(TailWide(METHOD, NM), "whole-wave cells: RhoT, 4+ materials");
     if (two_levels && *pk.l2_count <= wide_max) tail_wide<NM>(a, P, pk, lane);
   } else {
     // level 2: groups while its cells fit the GPU's resident waves at once;
@@ -425,7 +426,7 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixPar
     if constexpr (LEVEL == 2) {
       if (!two_levels) return;
       const int l2 = *pk.l2_count;
-      if (METHOD == PTE_RHOT && l2 <= wide_max) return;  // the whole-wave kernel's
+      if (TailWide(METHOD, NM) && l2 <= wide_max) return;  // the whole-wave kernel's
       grp = (long)l2 * G <= 64L * (long)resident;
     }
     const int lpc = grp ? G : 1;  // lanes per cell
@@ -546,7 +547,7 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   const int g1 = (pk.nwaves + G - 1) / G;
   const int g = g1 < cap ? g1 : cap;
   const bool l2 = G > 1 && pk.after2 > 0 && pk.l2_count &&
-                  P.line_search_max_iter <= G && (METHOD != PTE_RHOT || pk.l2_next);
+                  P.line_search_max_iter <= G && (!TailWide(METHOD, nm) || pk.l2_next);
   // level 2 on whole waves (RhoT): one wave per CU, so no two of the
   // kernel's long chains share a CU, each taking cells from the queue; in
   // groups: one block per wave slot, so that no block walks two slots that
@@ -564,7 +565,7 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
       pte_tail_kernel<K, METHOD, TailWavesL1<K>(), 1><<<g, 64, lds, s>>>(a, P, pk, cap,        \
                                                                          wide_max);           \
     } else {                                                                                   \
-      if constexpr (METHOD == PTE_RHOT)                                                        \
+      if constexpr (TailWide(METHOD, K))                                                       \
         pte_tail_kernel<K, METHOD, 1, 3><<<g3, 64, lds, s>>>(a, P, pk, 4 * ncu, wide_max);     \
       pte_tail_kernel<K, METHOD, 1, 2><<<pk.nwaves, 64, lds, s>>>(a, P, pk, 4 * ncu,           \
                                                                   wide_max);                   \
