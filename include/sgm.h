@@ -328,13 +328,14 @@ int sgm_driver_cache_info(sgm_ctx *ctx, long *out);
 
 /* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
 /* No reference counterpart: a plan bins the cells of a fixed CSR topology by
- * material count (2..6 run register-resident templated kernels) and by
- * material signature, autotunes block shape and cells-per-wave per bin, and
- * captures one hipGraph whose persistent kernel nodes solve every bin.
- * method: 0 = RhoT, 1 = PT, 2 = FixedT, 3 = FixedP (the per-cell sie_tot
- * array of execute then holds the cell temperature / pressure, and Tnorm is
- * ignored). Results are identical to sgm_pte_rhoT / sgm_pte_PT /
- * sgm_pte_fixedT / sgm_pte_fixedP on the same inputs. The plan keeps device copies of offsets and mat_ids. */
+ * material count (2..6 run register-resident templated kernels; 2..4 for
+ * RhoU) and by material signature, autotunes block shape and cells-per-wave
+ * per bin, and captures one hipGraph whose persistent kernel nodes solve
+ * every bin. method: 0 = RhoT, 1 = PT, 2 = FixedT, 3 = FixedP (the per-cell
+ * sie_tot array of execute then holds the cell temperature / pressure, and
+ * Tnorm is ignored), 4 = RhoU. Results are identical to sgm_pte_rhoT /
+ * sgm_pte_PT / sgm_pte_fixedT / sgm_pte_fixedP / sgm_pte_rhoU on the same
+ * inputs. The plan keeps device copies of offsets and mat_ids. */
 int sgm_plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
                     const uint8_t *mat_ids, int method, sgm_plan **plan, sgm_stream_t stream);
 /* autotune (optional; uses the given arrays as scratch-copied input) */

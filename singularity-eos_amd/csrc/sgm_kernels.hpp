@@ -60,7 +60,6 @@ struct ParkArgs {
 };
 
 // Closure formulations, by the method ids of the C ABI.
-// (RhoU runs on the plain and list kernels only; batched plans take 0..3)
 enum PteMethod {
   PTE_RHOT = 0,
   PTE_PT = 1,
@@ -71,13 +70,25 @@ enum PteMethod {
 };
 
 // Lanes per parked cell in the tail kernels: a group of 8 lanes runs each
-// cell's line-search trials side by side (pte_iterate_group); PT's trials
-// are not independent, so PT keeps one lane per cell.
+// cell's line-search trials side by side (pte_iterate_group); the trials of
+// PT (each P-T inversion starts from the previous trial's densities) and of
+// RhoU (each T(rho, e) inversion starts from the lambda cache the previous
+// trial left) are not independent, so those keep one lane per cell.
 #ifndef SGM_TAIL_GROUP_MIN_NM
 #define SGM_TAIL_GROUP_MIN_NM 2
 #endif
 constexpr int TailGroup(int method, int nm) {
-  return (method != PTE_PT && nm >= SGM_TAIL_GROUP_MIN_NM) ? 8 : 1;
+  return (method != PTE_PT && method != PTE_RHOU && nm >= SGM_TAIL_GROUP_MIN_NM) ? 8 : 1;
+}
+// The largest material count with a register-resident bin kernel per method:
+// RhoU's (2 nm)^2 Jacobian and per-material temperatures leave 5 and 6
+// materials to the list kernel.
+constexpr int BinMaxNM(int method) { return method == PTE_RHOU ? 4 : 6; }
+// FP64 words of one straggler record (a bound on ParkWordCount of the
+// method's solver: ParkWords for its equation count, plus RhoU's per-material
+// temperatures)
+constexpr int ParkStride(int method, int nm) {
+  return method == PTE_RHOU ? ParkWords(nm, 2 * nm) + nm : ParkWords(nm, nm + 1);
 }
 // Whether a bin's tail may run its second level on whole waves (RhoT from 4
 // materials: one material per lane pays for the wave's gathers only where
@@ -103,10 +114,8 @@ SGM_PTE_LAUNCHERS(rhot)
 SGM_PTE_LAUNCHERS(pt)
 SGM_PTE_LAUNCHERS(fixedt)
 SGM_PTE_LAUNCHERS(fixedp)
+SGM_PTE_LAUNCHERS(rhou)
 #undef SGM_PTE_LAUNCHERS
-hipError_t launch_pte_plain_rhou(const PteArgs &a, const MixParamsDev &P, hipStream_t s);
-hipError_t launch_pte_list_rhou(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
-                                const long *range, hipStream_t s);
 
 // the plain per-call kernel: one lane per cell, 1..8 materials
 inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParamsDev &P,
@@ -147,6 +156,8 @@ inline hipError_t launch_pte_bin(int method, int nm, const PteArgs &a, const Mix
     return launch_pte_bin_fixedt(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
   case PTE_FIXED_P:
     return launch_pte_bin_fixedp(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
+  case PTE_RHOU:
+    return launch_pte_bin_rhou(nm, a, P, cells, n, range, counter, block, cpw, grid, park, s);
   default: return hipErrorInvalidValue;
   }
 }
@@ -158,6 +169,7 @@ inline hipError_t launch_pte_tail(int method, int nm, const PteArgs &a, const Mi
   case PTE_PT: return launch_pte_tail_pt(nm, a, P, park, s);
   case PTE_FIXED_T: return launch_pte_tail_fixedt(nm, a, P, park, s);
   case PTE_FIXED_P: return launch_pte_tail_fixedp(nm, a, P, park, s);
+  case PTE_RHOU: return launch_pte_tail_rhou(nm, a, P, park, s);
   default: return hipErrorInvalidValue;
   }
 }

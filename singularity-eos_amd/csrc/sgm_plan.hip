@@ -157,7 +157,7 @@ hipError_t alloc_park(sgm_plan *p, const std::vector<unsigned long long> &keys) 
     Bin &b = p->bins[bi];
     if (b.nm < 2 || b.nm > 6 || nrec[bi] == 0) continue;
     ParkArgs &k = b.park;
-    k.stride = ParkWords(b.nm, b.nm + 1);
+    k.stride = ParkStride(p->method, b.nm);
     e = hipMalloc(&k.rec, sizeof(double) * k.stride * (size_t)nrec[bi]);
     if (e == hipSuccess) e = hipMalloc(&k.cell_of, sizeof(int) * nrec[bi]);
     k.l2_count = p->d_l2_counts + 2 * bi;
@@ -355,7 +355,7 @@ namespace sgm {
 int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
                 int method, int min_nm, sgm_plan **out, hipStream_t s) {
   if (!ctx || !out || ncell < 1 || nnz < 1 || nnz > 2147483647L || ncell > 2147483647L ||
-      !offsets || !mat_ids || method < 0 || method >= PTE_RHOU)  // plans take methods 0..3
+      !offsets || !mat_ids || method < 0 || method >= PTE_NMETHODS)
     return SGM_E_ARG;
   *out = nullptr;
   if (ctx->host_only) return SGM_E_NOGPU;
@@ -378,8 +378,8 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
     sgm_plan_destroy(p);
     return SGM_E_HIP;
   }
-  // bins: one per material count 2..6; everything else in one generic list
-  // bin per count (cells with fewer than min_nm materials in none)
+  // bins: one per material count 2..BinMaxNM(method); everything else in one
+  // generic list bin per count (cells with fewer than min_nm materials in none)
   long i = 0;
   while (i < ncell) {
     const int nm = (int)(keys[i] >> 56);
@@ -387,7 +387,7 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
     while (j < ncell && (int)(keys[j] >> 56) == nm) ++j;
     if (nm < min_nm) {
       // not solved
-    } else if (nm >= 2 && nm <= 6) {
+    } else if (nm >= 2 && nm <= BinMaxNM(method)) {
       Bin b{nm, i, j - i};
       p->bins.push_back(b);
     } else {

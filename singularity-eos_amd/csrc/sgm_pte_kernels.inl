@@ -515,12 +515,16 @@ hipError_t bin_launch(int nm, const PteArgs &a, const MixParamsDev &P, const int
   const ParkArgs pk = park ? *park : ParkArgs{};
 #define SGM_BIN(K)                                                                             \
   case K:                                                                                      \
-    static_assert(ParkWordCount<SysOf<K, METHOD, false>>() <= ParkWords(K, K + 1),             \
-                  "park record");                                                              \
-    if (pk.after > 0 && pk.stride < ParkWords(K, K + 1)) return hipErrorInvalidValue;          \
-    pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range,    \
-                                                       pk);                                    \
-    break;
+    if constexpr (K <= BinMaxNM(METHOD)) {                                                     \
+      static_assert(ParkWordCount<SysOf<K, METHOD, false>>() <= ParkStride(METHOD, K),         \
+                    "park record");                                                            \
+      if (pk.after > 0 && pk.stride < ParkStride(METHOD, K)) return hipErrorInvalidValue;      \
+      pte_bin_kernel<K, METHOD><<<grid, block, lds, s>>>(a, P, cells, n, counter, cpw, range,  \
+                                                         pk);                                  \
+      break;                                                                                   \
+    } else {                                                                                   \
+      return hipErrorInvalidValue;                                                             \
+    }
   switch (nm) {
     SGM_BIN(2)
     SGM_BIN(3)
@@ -537,7 +541,8 @@ template <int METHOD>
 hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const ParkArgs &pk,
                        hipStream_t s) {
   if (pk.nwaves <= 0) return hipSuccess;
-  if (pk.stride < ParkWords(nm, nm + 1)) return hipErrorInvalidValue;
+  if (nm < 2 || nm > BinMaxNM(METHOD) || pk.stride < ParkStride(METHOD, nm))
+    return hipErrorInvalidValue;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
   // persistent: at most as many waves as the GPU holds at the kernel's
   // occupancy (one wave per SIMD from 5 materials, else two)
@@ -561,7 +566,9 @@ hipError_t tail_launch(int nm, const PteArgs &a, const MixParamsDev &P, const Pa
   for (int level = 1; level <= (l2 ? 2 : 1); ++level) {
 #define SGM_TAIL(K)                                                                            \
   case K:                                                                                      \
-    if (level == 1) {                                                                          \
+    if constexpr (K > BinMaxNM(METHOD)) {                                                      \
+      return hipErrorInvalidValue;                                                             \
+    } else if (level == 1) {                                                                   \
       pte_tail_kernel<K, METHOD, TailWavesL1<K>(), 1><<<g, 64, lds, s>>>(a, P, pk, cap,        \
                                                                          wide_max);           \
     } else {                                                                                   \

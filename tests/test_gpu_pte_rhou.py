@@ -4,7 +4,8 @@ the oracle's restatement, through the C ABI.
 Bar: bit for bit -- per-material states (RhoU writes one temperature per
 material), convergence flags, iteration counts and error bits -- at the
 default and at the 1e-10 tolerances, on cells of the uneven 2..6-material
-workload and on 1, 7 and 8-material cells; batched plans refuse the method.
+workload and on 1, 7 and 8-material cells; batched plans (register-resident
+bins for 2..4 materials, the list kernel above) equal the plain path.
 """
 import numpy as np
 import pytest
@@ -61,11 +62,37 @@ def test_rhou_odd_material_counts(sgm, oracle, pool, torch_gpu):
     assert_identical(g, r, "rhou odd counts")
 
 
-def test_plan_refuses_rhou(sgm, workloads, pool, torch_gpu):
+@pytest.mark.parametrize("tight", [False, True])
+def test_rhou_batched_equals_plain_and_oracle(sgm, workloads, oracle, pool, torch_gpu, tight):
+    """Batched plans run RhoU in register-resident bins for 2..4 materials
+    and on the list kernel for 5 and 6; with straggler parking at several
+    thresholds (every cell parked after 1 iteration, the default 20, off)
+    every output, flag and iteration count equals the plain per-call path
+    and the oracle bit for bit."""
     torch = torch_gpu
-    ctx, _, _ = pool
-    a = workloads.uneven(ctx, 64, seed=0x5EED0000 + 62)
-    offs = torch.tensor(a["offsets"], device="cuda", dtype=torch.int32)
-    mats = torch.tensor(a["mat_ids"], device="cuda", dtype=torch.uint8)
-    with pytest.raises(RuntimeError):
-        sgm.Plan(ctx, a["ncell"], a["nnz"], offs, mats, 4)
+    ctx, o, _ = pool
+    a = workloads.uneven(ctx, 4000, seed=0x5EED0000 + 63 + int(tight))
+    params = sgm.mix_params(tight=tight)
+    g = run_plain(torch, ctx, a, 4, params)
+    r = run_oracle(oracle, o, a, 4, params.as_oracle_array())
+    assert_identical(g, r, "rhou plain")
+    from test_gpu_pte import FIELDS, dev
+    d = dev(torch, a)
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], 4)
+    bins = plan.describe()
+    assert sorted(b["nmat"] for b in bins if b["nmat"] >= 2) == [2, 3, 4]
+    assert sum(b["ncells"] for b in bins) == a["ncell"]
+    for after in (1, 20, 0):
+        plan.set_straggler_threshold(after)
+        w = {k: d[k].clone() for k in FIELDS}
+        for k in ("conv", "niter", "resid", "err"):
+            d[k].fill_(-7)
+        assert plan.execute(w["rho"], w["vfrac"], w["sie"], w["temp"], w["press"],
+                            d["vfrac_tot"], d["sie_tot"], params=params, conv=d["conv"],
+                            niter=d["niter"], resid=d["resid"], err=d["err"]) == 0
+        torch.cuda.synchronize()
+        out = {k: w[k].cpu().numpy() for k in FIELDS}
+        out.update({k: d[k].cpu().numpy() for k in ("conv", "niter", "resid", "err")})
+        assert_identical(out, g, f"rhou batched after={after}")
+        assert np.array_equal(out["err"], g["err"])
+    plan.close()
