@@ -186,7 +186,10 @@ int sgm_vec_rhoe_from_PT(sgm_ctx *ctx, int matindex, const double *P, const doub
 /* ---- PTE closure (plain per-call path) ----------------------------------- */
 /* One launch over a CSR list of mixed cells: cell c owns entries
  * [offsets[c], offsets[c+1]) of the per-(cell,material) arrays; mat_ids holds
- * context material indices; 1..8 materials per cell. Replaces constructing
+ * context material indices; 1..64 materials per cell (1..8 one lane per cell
+ * with the state in registers, 9..64 one wave per cell with the Newton system
+ * in LDS; 9+ material cells of sgm_pte_rhoU are flagged SGM_ERRB_INPUT, its
+ * formulation having no one-wave form). Replaces constructing
  * PTESolverRhoT / PTESolverPT(nmat, eos, vfrac_tot, sie_tot, rho, vfrac, sie,
  * temp, press, lambda, scratch, Tnorm, params) and calling PTESolver
  * (mixed_cell_models.hpp:922-935, 1245-1253, 2188-2291) once per cell.
@@ -238,7 +241,7 @@ int sgm_pte_fixedP(sgm_ctx *ctx, long ncell, long nnz, const int *offsets,
  * energy densities of every material are solved for, so that the materials
  * share one pressure and one temperature (each from T(rho, e) of its EOS).
  * Arguments as sgm_pte_rhoT; outputs overwrite rho, vfrac, sie, temp (per
- * material), press. Plain per-call kernels only (no batched plan). */
+ * material), press. Batched plans take method 4 (bins for 2..4 materials). */
 int sgm_pte_rhoU(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const uint8_t *mat_ids,
                  double *rho, double *vfrac, double *sie, double *temp, double *press,
                  const double *vfrac_tot, const double *sie_tot, const double *Tnorm,
@@ -280,8 +283,8 @@ int sgm_approx_temperature(sgm_ctx *ctx, long ncell, long nnz, const int *offset
  * Differences: each cell's lambda cache has storage of its own (the
  * reference places it inside the PTE solver scratch, where with three or
  * more materials it overlaps the solver's last arrays; only tabulated EOS
- * write the cache, so results without tables are the same); cells with more
- * than 8 participating materials are not solved (pte_converged 0). The
+ * write the cache, so results without tables are the same). Cells of 9..64
+ * participating materials are solved one wave per cell. The
  * context keeps the driver's work buffers and the batched plan of the last
  * call's cell topology until sgm_finalize, and reuses the plan while calls
  * bring the same topology; calls on one context are serialized, and each
@@ -329,7 +332,8 @@ int sgm_driver_cache_info(sgm_ctx *ctx, long *out);
 /* ---- PTE closure (batched, persistent, hipGraph path) -------------------- */
 /* No reference counterpart: a plan bins the cells of a fixed CSR topology by
  * material count (2..6 run register-resident templated kernels; 2..4 for
- * RhoU) and by material signature, autotunes block shape and cells-per-wave
+ * RhoU; 9..64 one wave per cell) and by material signature, autotunes block
+ * shape and cells-per-wave
  * per bin, and captures one hipGraph whose persistent kernel nodes solve
  * every bin. method: 0 = RhoT, 1 = PT, 2 = FixedT, 3 = FixedP (the per-cell
  * sie_tot array of execute then holds the cell temperature / pressure, and

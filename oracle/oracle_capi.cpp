@@ -319,35 +319,46 @@ long sgo_pte_solve(int method, long ncell, const int *offsets, const int *mat_eo
   long nfail = 0;
 #pragma omp parallel num_threads(threads_or_default(nthreads)) reduction(+ : nfail)
   {
-    std::vector<Real> scratch(PTESolverFixedPRequiredScratch(8) + PTESolverRhoTRequiredScratch(8) +
-                              PTESolverRhoURequiredScratch(8));
-    EOS eos_cell[8];
+    // cells of up to 64 materials (the device's one-wave-per-cell limit)
+    constexpr int kMaxMats = 64;
+    std::vector<Real> scratch(PTESolverFixedPRequiredScratch(kMaxMats) +
+                              PTESolverRhoTRequiredScratch(kMaxMats) +
+                              PTESolverRhoURequiredScratch(kMaxMats));
+    std::vector<EOS> eos_cell(kMaxMats);
 #pragma omp for schedule(dynamic, 256)
     for (long c = 0; c < ncell; ++c) {
       const int off = offsets[c];
       const int nmat = offsets[c + 1] - off;
+      if (nmat < 1 || nmat > kMaxMats) {
+        if (out_converged) out_converged[c] = 0;
+        if (out_niter) out_niter[c] = 0;
+        if (out_residual) out_residual[c] = 0;
+        if (out_err) out_err[c] = 32;  // malformed cell
+        nfail += 1;
+        continue;
+      }
       for (int m = 0; m < nmat; ++m) eos_cell[m] = pool[mat_eos[off + m]];
       Real *lam = lambda ? lambda + 3L * off : nullptr;
       const Real tn = Tnorm ? Tnorm[c] : 0.0;
       SolverStatus st;
       if (method == 0) {
-        PTERhoT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+        PTERhoT s(nmat, eos_cell.data(), vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                   temp + off, press + off, lam, scratch.data(), tn, params);
         st = PTESolver(s);
       } else if (method == 1) {
-        PTEPT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+        PTEPT s(nmat, eos_cell.data(), vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                 temp + off, press + off, lam, scratch.data(), tn, params);
         st = PTESolver(s);
       } else if (method == 4) {
-        PTERhoU s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+        PTERhoU s(nmat, eos_cell.data(), vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                   temp + off, press + off, lam, scratch.data(), tn, params);
         st = PTESolver(s);
       } else if (method == 2) {
-        PTEFixedT s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+        PTEFixedT s(nmat, eos_cell.data(), vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                     temp + off, press + off, lam, scratch.data(), params);
         st = PTESolver(s);
       } else {
-        PTEFixedP s(nmat, eos_cell, vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
+        PTEFixedP s(nmat, eos_cell.data(), vfrac_tot[c], cell_in[c], rho + off, vfrac + off, sie + off,
                     temp + off, press + off, lam, scratch.data(), params);
         st = PTESolver(s);
       }

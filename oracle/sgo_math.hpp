@@ -452,7 +452,7 @@ inline Real sum_neumaier(const Real *data, int n, int offset = 0, int iskip = -1
 // squared norm falls below (eps*max_norm)^2/n*(n-k) are treated as zero and
 // their unknowns set to 0). Matrix is row-major n x n.
 inline void qr_colpiv_solve(const int n, Real *a, Real *b, Real *x) {
-  constexpr int NMAX = 16;  // neq <= 2 * 8 materials
+  constexpr int NMAX = 128;  // neq <= 2 * 64 materials
   int perm[NMAX];
   Real cn_upd[NMAX], cn_dir[NMAX];
   for (int j = 0; j < n; ++j) {
@@ -571,7 +571,7 @@ inline bool solve_Ax_b_wscr(const int n, Real *a, Real *b, Real *scr) {
     b[0] = x[0];
     b[1] = x[1];
   } else {
-    Real x[16];
+    Real x[128];
     qr_colpiv_solve(n, a, b, x);
     for (int i = 0; i < n; ++i) b[i] = x[i];
   }

@@ -5,12 +5,16 @@
 #pragma once
 
 #include "sgm_pte.hpp"
+#include "sgm_pte_wave.hpp"
 
 #include <hip/hip_runtime.h>
 
 namespace sgm {
 
+// materials per cell of the register-resident solvers (one lane per cell);
+// cells of up to MAX_MATS_WAVE materials take one wave per cell
 constexpr int MAX_MATS_PER_CELL = 8;
+constexpr int MAX_MATS_WAVE = wave::MAX_MATS;
 constexpr int MAX_REGISTRY = 64;
 // status bit of a malformed cell (SGM_ERRB_INPUT of the C ABI)
 constexpr int ERRB_INPUT = 32;
@@ -96,13 +100,19 @@ constexpr int ParkStride(int method, int nm) {
 constexpr bool TailWide(int method, int nm) { return method == PTE_RHOT && nm >= 4; }
 
 // Per-method launchers, one translation unit pair per method
-// (sgm_pte_plain_<name>.hip: plain + list kernels; sgm_pte_bins_<name>.hip:
-// the batched bin kernels for 2..6 materials). range: NULL, or device
+// (sgm_pte_plain_<name>.hip: plain, list and wave kernels;
+// sgm_pte_bins_<name>.hip: the batched bin kernels). range: NULL, or device
 // {start, count} bounds into `cells` read by the kernel (n is then the host's
-// upper bound on count, used to size the list kernel's grid).
+// upper bound on count, used to size the grid). The plain launcher runs the
+// plain kernel and then the wave kernel over every cell; the list launcher
+// the list kernel only; the wave launcher the wave kernel, which solves the
+// listed cells of MAX_MATS_PER_CELL + 1 .. MAX_MATS_WAVE materials and
+// leaves the others alone.
 #define SGM_PTE_LAUNCHERS(name)                                                                \
   hipError_t launch_pte_plain_##name(const PteArgs &a, const MixParamsDev &P, hipStream_t s);  \
   hipError_t launch_pte_list_##name(const PteArgs &a, const MixParamsDev &P, const int *cells, \
+                                    long n, const long *range, hipStream_t s);                 \
+  hipError_t launch_pte_wave_##name(const PteArgs &a, const MixParamsDev &P, const int *cells, \
                                     long n, const long *range, hipStream_t s);                 \
   hipError_t launch_pte_bin_##name(int nm, const PteArgs &a, const MixParamsDev &P,            \
                                    const int *cells, long n, const long *range, int *counter,  \
@@ -117,7 +127,8 @@ SGM_PTE_LAUNCHERS(fixedp)
 SGM_PTE_LAUNCHERS(rhou)
 #undef SGM_PTE_LAUNCHERS
 
-// the plain per-call kernel: one lane per cell, 1..8 materials
+// the plain per-call kernels: one lane per cell of 1..8 materials, one wave
+// per cell of 9..64
 inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParamsDev &P,
                                    hipStream_t s) {
   switch (method) {
@@ -129,18 +140,36 @@ inline hipError_t launch_pte_plain(int method, const PteArgs &a, const MixParams
   default: return hipErrorInvalidValue;
   }
 }
-// the plain solver over an explicit cell list
-inline hipError_t launch_pte_list(int method, const PteArgs &a, const MixParamsDev &P,
+// the one-wave-per-cell solver over an explicit cell list (cells of
+// 9..64 materials; the others are left alone)
+inline hipError_t launch_pte_wave(int method, const PteArgs &a, const MixParamsDev &P,
                                   const int *cells, long n, hipStream_t s,
                                   const long *range = nullptr) {
   switch (method) {
-  case PTE_RHOT: return launch_pte_list_rhot(a, P, cells, n, range, s);
-  case PTE_PT: return launch_pte_list_pt(a, P, cells, n, range, s);
-  case PTE_FIXED_T: return launch_pte_list_fixedt(a, P, cells, n, range, s);
-  case PTE_FIXED_P: return launch_pte_list_fixedp(a, P, cells, n, range, s);
-  case PTE_RHOU: return launch_pte_list_rhou(a, P, cells, n, range, s);
+  case PTE_RHOT: return launch_pte_wave_rhot(a, P, cells, n, range, s);
+  case PTE_PT: return launch_pte_wave_pt(a, P, cells, n, range, s);
+  case PTE_FIXED_T: return launch_pte_wave_fixedt(a, P, cells, n, range, s);
+  case PTE_FIXED_P: return launch_pte_wave_fixedp(a, P, cells, n, range, s);
+  case PTE_RHOU: return launch_pte_wave_rhou(a, P, cells, n, range, s);
   default: return hipErrorInvalidValue;
   }
+}
+// the plain solvers over an explicit cell list: one lane per cell of 1..8
+// materials, then (wide) one wave per cell of 9..64
+inline hipError_t launch_pte_list(int method, const PteArgs &a, const MixParamsDev &P,
+                                  const int *cells, long n, hipStream_t s,
+                                  const long *range = nullptr, bool wide = true) {
+  hipError_t e;
+  switch (method) {
+  case PTE_RHOT: e = launch_pte_list_rhot(a, P, cells, n, range, s); break;
+  case PTE_PT: e = launch_pte_list_pt(a, P, cells, n, range, s); break;
+  case PTE_FIXED_T: e = launch_pte_list_fixedt(a, P, cells, n, range, s); break;
+  case PTE_FIXED_P: e = launch_pte_list_fixedp(a, P, cells, n, range, s); break;
+  case PTE_RHOU: e = launch_pte_list_rhou(a, P, cells, n, range, s); break;
+  default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess || !wide) return e;
+  return launch_pte_wave(method, a, P, cells, n, s, range);
 }
 // batched persistent bin (2..6 materials): cells all have nm materials
 inline hipError_t launch_pte_bin(int method, int nm, const PteArgs &a, const MixParamsDev &P,

@@ -23,6 +23,7 @@ struct Bin {
   int grid = 0;
   ParkArgs park;   // straggler records (register-resident bins only)
   int nruns = 0;   // signature runs in the bin
+  bool wide = false;  // list bin of the cells of 9..64 materials (one wave per cell)
 };
 
 struct GraphKey {
@@ -201,13 +202,21 @@ hipError_t reset_counters(sgm_plan *p, hipStream_t s) {
   return e;
 }
 
+// A list bin: the plain solver over its cells, one lane per cell, or one
+// wave per cell for the bin of 9..64 materials.
+hipError_t launch_list_bin(const sgm_plan *p, const Bin &b, const PteArgs &a,
+                           const MixParamsDev &P, const int *cells, hipStream_t s) {
+  if (b.wide) return launch_pte_wave(p->method, a, P, cells, b.n, s);
+  return launch_pte_list(p->method, a, P, cells, b.n, s, nullptr, false);
+}
+
 // One bin: the persistent kernel, then (with parking) the in-order listing of
 // its parked cells and the tail kernel that finishes them.
 hipError_t launch_one_bin(sgm_plan *p, size_t i, const PteArgs &a, const MixParamsDev &P,
                           hipStream_t s) {
   const Bin &b = p->bins[i];
   const int *cells = p->d_sorted + b.start;
-  if (b.nm < 2 || b.nm > 6) return launch_pte_list(p->method, a, P, cells, b.n, s);
+  if (b.nm < 2 || b.nm > 6) return launch_list_bin(p, b, a, P, cells, s);
   ParkArgs pk = b.park;
   pk.after = pk.rec ? p->park_after : 0;
   hipError_t e = launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + i, b.block,
@@ -252,7 +261,7 @@ hipError_t launch_bins(sgm_plan *p, const PteArgs &a, const MixParamsDev &P, hip
     const Bin &b = p->bins[i];
     const int *cells = p->d_sorted + b.start;
     if (b.nm < 2 || b.nm > 6) {
-      if ((e = launch_pte_list(p->method, a, P, cells, b.n, s)) != hipSuccess) return e;
+      if ((e = launch_list_bin(p, b, a, P, cells, s)) != hipSuccess) return e;
       continue;
     }
     ParkArgs pk = b.park;
@@ -391,9 +400,11 @@ int plan_create(sgm_ctx *ctx, long ncell, long nnz, const int *offsets, const ui
       Bin b{nm, i, j - i};
       p->bins.push_back(b);
     } else {
-      // sorted keys put counts 1 before 2..6 and 7, 8, invalid after; keep
-      // each such run as its own list bin
+      // sorted keys put counts 1 before 2..6 and 7, 8, modified, 9..64
+      // (count field 0x80), invalid after; keep each such run as its own
+      // list bin
       Bin b{0, i, j - i};
+      b.wide = nm == 0x80;
       p->bins.push_back(b);
     }
     i = j;
@@ -767,7 +778,7 @@ int sgm_plan_execute_bin(sgm_plan *p, int bin, double *rho, double *vfrac, doubl
   pk.after = (b.nm >= 2 && b.nm <= 6 && pk.rec) ? p->park_after : 0;
   if (e == hipSuccess)
     e = (b.nm < 2 || b.nm > 6)
-            ? launch_pte_list(p->method, a, P, cells, b.n, s)
+            ? launch_list_bin(p, b, a, P, cells, s)
             : launch_pte_bin(p->method, b.nm, a, P, cells, b.n, p->d_counters + bin, b.block,
                              b.cpw, b.grid, s, nullptr, &pk);
   if (e == hipSuccess) e = hipEventRecord(ev[1], s);

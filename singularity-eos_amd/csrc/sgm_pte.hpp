@@ -256,6 +256,36 @@ SGM_HD const EosDev &registry_entry(int i) {
 #endif
 }
 
+// GetSieCvFromTAndPreferred (mixed_cell_models.hpp:599-620) for one material
+template <bool MODS>
+SGM_HD void sie_cv_from_T_preferred(const EosDev &e, Lam &l, Real r, Real Pin, Real T, Real &s,
+                                    Real &cv) {
+  using Ev = sgm::Ev<MODS>;
+  if (Ev::preferred_input(e) & tq::pressure) {
+    // IdealGas (modified or not) is the only model here whose preferred
+    // input includes pressure: MinimumPressure() = 0,
+    // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549), its
+    // closed-form P-T inversion (eos_ideal.hpp:201-207) under the
+    // modifiers' DensityEnergyFromPressureTemperature (shifted_eos.hpp:399-403,
+    // scaled_eos.hpp:328-335), inlined: no call takes this cell's state
+    // by reference
+    const Real Pmin = 0.0, Pmax = 1e100;
+    const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
+    ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
+    if (Ev::modified(e)) {
+      if (e.mods & MOD_SHIFTED) s = s + e.shift;
+      if (e.mods & MOD_SCALED) {
+        r = r * e.inv_scale;
+        s = s * e.scale;
+      }
+    }
+    cv = Ev::cv_re(e, r, s, l);
+  } else {
+    s = Ev::e_rT(e, r, T, l);
+    cv = Ev::cv_rT(e, r, T, l);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Shared base state (PTESolverBase)
 // ---------------------------------------------------------------------------
@@ -338,30 +368,7 @@ struct PteBase {
   }
   // :599-620
   SGM_HD void SieCvFromTAndPreferred(int m, Real r, Real Pin, Real T, Real &s, Real &cv) {
-    const EosDev &e = E(m);
-    if (Ev::preferred_input(e) & tq::pressure) {
-      // IdealGas (modified or not) is the only model here whose preferred
-      // input includes pressure: MinimumPressure() = 0,
-      // MaximumPressureAtTemperature() = 1e100 (eos_base.hpp:544-549), its
-      // closed-form P-T inversion (eos_ideal.hpp:201-207) under the
-      // modifiers' DensityEnergyFromPressureTemperature (shifted_eos.hpp:399-403,
-      // scaled_eos.hpp:328-335), inlined: no call takes this cell's state
-      // by reference
-      const Real Pmin = 0.0, Pmax = 1e100;
-      const Real Pguess = bad_value(Pin) ? 0.5 * (Pmin + Pmax) : smax(Pmin, smin(Pmax, Pin));
-      ideal::rhoe_PT(e.m.p, Pguess, T, r, s);
-      if (Ev::modified(e)) {
-        if (e.mods & MOD_SHIFTED) s = s + e.shift;
-        if (e.mods & MOD_SCALED) {
-          r = r * e.inv_scale;
-          s = s * e.scale;
-        }
-      }
-      cv = Ev::cv_re(e, r, s, lam[m]);
-    } else {
-      s = Ev::e_rT(e, r, T, lam[m]);
-      cv = Ev::cv_rT(e, r, T, lam[m]);
-    }
+    sie_cv_from_T_preferred<MODS>(E(m), lam[m], r, Pin, T, s, cv);
   }
   // :500-581; Tin are the caller's input temperatures
   SGM_HD Real GetTguess(const Real (&Tin)[NM]) {
@@ -1658,28 +1665,39 @@ constexpr int ParkWordCount() {
   return k;
 }
 
-template <int NM, class S>
-SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &st) {
+// the loop state before the first pass; err: the residual of the initial state
+SGM_HD void driver_start(DriverState &d, Real err, int max_iter) {
+  d.err = err;
+  d.converged = false;
+  d.close_enough = false;
+  d.max_iter = max_iter;
+  d.small_step_iters = 0;
+  d.niter = 0;
+  d.max_line_niter = 0;
+  d.slow = false;
+}
+SGM_HD void status_start(CellStatus &st) {
   st.converged = 0;
   st.niter = 0;
   // set by pte_end; defined here so that a parked state (and the fixed-point
   // probe's comparison of it) never holds an indeterminate word
   st.err = 0;
   st.residual = 0;
-  d.err = s.Init(Tin);
-  d.converged = false;
-  d.close_enough = false;
-  d.max_iter = NM * s.P.pte_max_iter_per_mat;
-  d.small_step_iters = 0;
-  d.niter = 0;
-  d.max_line_niter = 0;
-  d.slow = false;
+}
+
+template <int NM, class S>
+SGM_HD void pte_begin(S &s, const Real (&Tin)[NM], DriverState &d, CellStatus &st) {
+  status_start(st);
+  const Real err = s.Init(Tin);
+  driver_start(d, err, NM * s.P.pte_max_iter_per_mat);
 }
 
 // one pass of `for (niter = 0; niter < max_iter; ++niter)`; true once the
-// loop has ended (break or exhausted)
-template <int NM, class S>
-SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
+// loop has ended (break or exhausted). Any solver with the formulation
+// interface (CheckPTE, Jacobian, Solve, ScaleDx, TestUpdate, Fixup, err, P):
+// the register-resident ones below and the one-wave-per-cell ones.
+template <class S>
+SGM_HD bool pte_step(S &s, DriverState &d, CellStatus &st) {
   const MixParamsDev &p = s.P;
   if (d.niter >= d.max_iter) return true;
   st.niter = d.niter;
@@ -1740,6 +1758,10 @@ SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
   }
   ++d.niter;
   return d.niter >= d.max_iter;
+}
+template <int NM, class S>
+SGM_HD bool pte_iterate(S &s, DriverState &d, CellStatus &st) {
+  return pte_step(s, d, st);
 }
 
 // Every lane of a group takes lane `src`'s copy of the cell state.

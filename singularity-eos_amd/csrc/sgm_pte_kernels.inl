@@ -11,10 +11,12 @@
 //    EOS dispatch is wave-uniform); lanes retire finished cells and take new
 //    ones between Newton iterations, from a per-wave reserve refilled
 //    `chunk` (>= 64) cells at a time from a device counter.
+//  * pte_wave_kernel: cells of 9..64 materials, one wave per cell
+//    (sgm_pte_wave.hpp), after the plain kernel and beside the list kernel.
 //
-// Both stage the material registry once per block into LDS (the per-material
+// All stage the material registry once per block into LDS (the per-material
 // parameters every EOS evaluation touches) and keep all per-cell state in
-// VGPRs. Per cell: reads rho, vfrac, temp, press per material + sie_tot,
+// VGPRs (the wave kernel: the Newton system in LDS). Per cell: reads rho, vfrac, temp, press per material + sie_tot,
 // vfrac_tot (+Tnorm); writes rho, vfrac, sie, temp, press + status.
 #pragma once
 #include "sgm_kernels.hpp"
@@ -122,12 +124,12 @@ __device__ __attribute__((noinline)) void solve_cell_call(const EosDev *lds, con
   solve_cell<NM, METHOD>(lds, a, P, c, off);
 }
 
-// validates a cell's CSR slice; returns nmat or -1
+// validates a cell's CSR slice; returns nmat (1..MAX_MATS_WAVE) or -1
 __device__ __forceinline__ int cell_slice(const PteArgs &a, long c, int &off) {
   const long o0 = a.offsets[c], o1 = a.offsets[c + 1];
   if (!(o0 >= 0 && o1 >= o0 && o1 <= a.nnz)) return -1;
   const int nm = (int)(o1 - o0);
-  if (nm < 1 || nm > MAX_MATS_PER_CELL) return -1;
+  if (nm < 1 || nm > MAX_MATS_WAVE) return -1;
   for (int m = 0; m < nm; ++m)
     if (a.mat_ids[o0 + m] >= a.nreg) return -1;
   off = (int)o0;
@@ -158,7 +160,10 @@ __global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev 
     case 6: solve_cell_call<6, METHOD>(lds, a, P, c, off); break;
     case 7: solve_cell_call<7, METHOD>(lds, a, P, c, off); break;
     case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
-    default: mark_bad(a, c); break;
+    default:
+      // cells of 9..64 materials are the wave kernel's
+      if (!(nm > MAX_MATS_PER_CELL && wave::HasWaveForm(METHOD))) mark_bad(a, c);
+      break;
     }
   }
 }
@@ -496,6 +501,10 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) pte_tail_kernel(PteArgs a, MixPar
 }
 
 template <int METHOD>
+hipError_t wave_launch(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                       hipStream_t s, const long *range = nullptr);
+
+template <int METHOD>
 hipError_t plain_launch(const PteArgs &a, const MixParamsDev &P, hipStream_t s) {
   if (a.ncell <= 0) return hipSuccess;
   const size_t lds = (size_t)a.nreg * sizeof(EosDev);
@@ -503,7 +512,9 @@ hipError_t plain_launch(const PteArgs &a, const MixParamsDev &P, hipStream_t s) 
   const long cap = (long)device_cu_count() * 32;
   const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
   pte_plain_kernel<METHOD><<<g, 256, lds, s>>>(a, P);
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return wave_launch<METHOD>(a, P, nullptr, a.ncell, s);
 }
 
 template <int METHOD>
@@ -625,7 +636,10 @@ __global__ void __launch_bounds__(256) pte_list_kernel(PteArgs a, MixParamsDev P
     case 6: solve_cell_call<6, METHOD>(lds, a, P, c, off); break;
     case 7: solve_cell_call<7, METHOD>(lds, a, P, c, off); break;
     case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
-    default: mark_bad(a, c); break;
+    default:
+      // cells of 9..64 materials are the wave kernel's
+      if (!(nm > MAX_MATS_PER_CELL && wave::HasWaveForm(METHOD))) mark_bad(a, c);
+      break;
     }
   }
 }
@@ -640,6 +654,116 @@ hipError_t list_launch(const PteArgs &a, const MixParamsDev &P, const int *cells
   const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
   pte_list_kernel<METHOD><<<g, 256, lds, s>>>(a, P, cells, n, range);
   return hipGetLastError();
+}
+
+// One cell of nm (9..64) materials on the whole wave: lane m loads material
+// m, the wave runs the driver loop (pte_step) on the one-wave formulation,
+// lane m stores material m and lane 0 the cell's status.
+template <int METHOD>
+__device__ __forceinline__ void wave_solve_cell(const PteArgs &a, const MixParamsDev &P, long c,
+                                                int off, int nm, Real *work, int lane) {
+  using Sys = wave::SysOf<METHOD>;
+  Sys s;
+  s.Setup(nm, Sys::Equations(nm), lane, work);
+  const long k = off + (lane < nm ? lane : 0);
+  s.eidx = a.mat_ids[k];
+  s.rho = a.rho[k];
+  s.vfrac = a.vfrac[k];
+  s.press = a.press[k];
+  const Real Tin = a.temp[k];
+  if (a.lambda) {
+    s.lam.present = true;
+    s.lam.lRho = a.lambda[3L * k + 0];
+    s.lam.lT = a.lambda[3L * k + 1];
+  } else {
+    s.lam.present = false;
+  }
+  s.SetCellInputs(a.vfrac_tot[c], a.sie_tot[c], a.Tnorm ? a.Tnorm[c] : 0.0);
+  s.err = 0;
+  s.P = P;
+  CellStatus st;
+  DriverState d;
+  status_start(st);
+  const Real err0 = s.Init(Tin);
+  driver_start(d, err0, nm * P.pte_max_iter_per_mat);
+  while (!wave::uni(pte_step(s, d, st))) {
+  }
+  s.Fixup();
+  st.converged = d.converged ? 1 : 0;
+  st.residual = d.err;
+  st.err = s.ErrAll();
+  if (lane < nm) {
+    a.rho[k] = s.rho;
+    a.vfrac[k] = s.vfrac;
+    a.sie[k] = s.sie;
+    if (Sys::kStoresTemp) a.temp[k] = s.Tout();
+    a.press[k] = s.Pout();
+    if (a.lambda) {
+      a.lambda[3L * k + 0] = s.lam.lRho;
+      a.lambda[3L * k + 1] = s.lam.lT;
+    }
+  }
+  if (lane == 0) {
+    if (a.conv) a.conv[c] = st.converged;
+    if (a.niter) a.niter[c] = st.niter;
+    if (a.resid) a.resid[c] = st.residual;
+    if (a.err) a.err[c] = st.err;
+  }
+}
+
+// Cells of 9..64 materials, one wave (the whole 64-lane workgroup) per cell:
+// each wave reads 64 list entries at a time and solves the wide cells among
+// them one after another; the others it leaves to the plain and list
+// kernels. A wave that finds none never stages the registry. LDS: the
+// registry, then the wave's Newton system. cells == nullptr: cells 0..n-1.
+template <int METHOD>
+__global__ void __launch_bounds__(64) pte_wave_kernel(PteArgs a, MixParamsDev P,
+                                                      const int *__restrict__ cells, long n,
+                                                      const long *range) {
+  extern __shared__ EosDev lds[];
+  if (range) {
+    cells += range[0];
+    n = range[1];
+  }
+  const int lane = threadIdx.x;
+  Real *work = reinterpret_cast<Real *>(lds + a.nreg);
+  bool staged = false;
+  for (long j0 = (long)blockIdx.x * 64; j0 < n; j0 += (long)gridDim.x * 64) {
+    const long j = j0 + lane;
+    int c = -1, nm = 0, off = 0;
+    if (j < n) {
+      c = cells ? cells[j] : (int)j;
+      if (c >= 0 && c < a.ncell) nm = cell_slice(a, c, off);
+    }
+    unsigned long long wide = __ballot(nm > MAX_MATS_PER_CELL);
+    if (!wide) continue;
+    if (!staged) {
+      stage_registry(lds, a.d_eos, a.nreg);
+      staged = true;
+    }
+    while (wide) {
+      const int src = __builtin_ctzll(wide);
+      wide &= wide - 1;
+      wave_solve_cell<METHOD>(a, P, wave::at(c, src), wave::at(off, src), wave::at(nm, src),
+                              work, lane);
+    }
+  }
+}
+
+template <int METHOD>
+hipError_t wave_launch(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                       hipStream_t s, const long *range) {
+  if constexpr (!wave::HasWaveForm(METHOD)) {
+    return hipSuccess;  // the plain and list kernels flag such cells
+  } else {
+    if (n <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.nreg * sizeof(EosDev) + sizeof(Real) * wave::WORK_WORDS;
+    const long blocks_needed = (n + 63) / 64;
+    const long cap = (long)device_cu_count() * 8;
+    const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
+    pte_wave_kernel<METHOD><<<g, 64, lds, s>>>(a, P, cells, n, range);
+    return hipGetLastError();
+  }
 }
 } // namespace kern
 } // namespace sgm

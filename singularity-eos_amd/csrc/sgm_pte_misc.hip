@@ -15,7 +15,8 @@ __global__ void zero_ints_kernel(int *p, int n) {
 // 55 is free). With a cost array, bit 55 is set on cells below the
 // threshold: within a material count, the costly cells sort first.
 // Cells of two or more materials that hold a material with modifiers get
-// count field nm | 0x40: past the register-resident bins.
+// count field nm | 0x40: past the register-resident bins; cells of 9..64
+// materials (the one-wave-per-cell kernel's) count field 0x80.
 __global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long ncell, long nnz,
                                  const int *cost, int threshold, unsigned long long mod_mask,
                                  unsigned long long *keys, int *ids) {
@@ -31,6 +32,9 @@ __global__ void plan_keys_kernel(const int *offsets, const uint8_t *mat, long nc
     k = (unsigned long long)((nm >= 2 && modified) ? (nm | 0x40) : nm) << 56;
     for (int m = 0; m < nm && m < 7; ++m) k |= (unsigned long long)mat[o0 + m] << (48 - 8 * m);
     if (cost && cost[c] < threshold) k |= 1ull << 55;
+  } else if (o0 >= 0 && o1 >= o0 && o1 <= nnz && o1 - o0 <= MAX_MATS_WAVE) {
+    k = 0x80ull << 56;
+    for (int m = 0; m < 7; ++m) k |= (unsigned long long)mat[o0 + m] << (48 - 8 * m);
   } else {
     k = 0xFFull << 56;
   }

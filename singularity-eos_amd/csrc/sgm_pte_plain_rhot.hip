@@ -8,4 +8,8 @@ hipError_t launch_pte_list_rhot(const PteArgs &a, const MixParamsDev &P, const i
                               const long *range, hipStream_t s) {
   return kern::list_launch<0>(a, P, cells, n, s, range);
 }
+hipError_t launch_pte_wave_rhot(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
+                              const long *range, hipStream_t s) {
+  return kern::wave_launch<0>(a, P, cells, n, s, range);
+}
 } // namespace sgm
