@@ -178,7 +178,8 @@ def test_batched_graph_replay_and_autotune_invariance(sgm, workloads, pool, torc
 
 
 def test_odd_material_counts_and_bad_cells(sgm, oracle, pool, torch_gpu):
-    """1, 7 and 8 materials take the generic path; malformed cells are
+    """1, 7 and 8 materials take the generic path; malformed cells (offsets
+    out of order, more than 64 materials, a material the context lacks) are
     flagged SGM_ERRB_INPUT without touching memory outside their slice."""
     ctx, o, _ = pool
     rng = np.random.default_rng(5)
@@ -209,17 +210,18 @@ def test_odd_material_counts_and_bad_cells(sgm, oracle, pool, torch_gpu):
     assert_identical(g, r, "odd counts")
     b, _ = run_batched(torch_gpu, sgm, ctx, A, 0, params)
     assert_identical(b, g, "odd counts batched")
-    # malformed: decreasing offsets, 9 materials, unknown material index
+    # malformed: decreasing offsets, 68 materials (past the 64 of one wave),
+    # unknown material index; the 9-material cell beside them is solved
     B = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in A.items()}
-    B["offsets"] = np.array([0, 3, 2, 11, 20, 22, 23, 31], dtype=np.int32)
-    B["nnz"] = 31
+    B["offsets"] = np.array([0, 3, 2, 70, 79, 81, 82, 90], dtype=np.int32)
+    B["nnz"] = 90
     for k in ("rho", "vfrac", "sie", "temp", "press"):
-        B[k] = np.resize(A[k], 31)
-    B["mat_ids"] = np.resize(A["mat_ids"], 31)
-    B["mat_ids"][20] = 60  # cell 4 names a material the context does not hold
+        B[k] = np.resize(A[k], 90)
+    B["mat_ids"] = np.resize(A["mat_ids"], 90)
+    B["mat_ids"][79] = 60  # cell 4 names a material the context does not hold
     g = run_plain(torch_gpu, ctx, B, 0, params)
     assert g["err"][1] & 32 and g["err"][2] & 32 and g["err"][4] & 32
-    assert g["err"][0] & 32 == 0
+    assert g["err"][0] & 32 == 0 and g["err"][3] & 32 == 0
 
 
 def test_empty_and_rejected_calls(sgm, pool, torch_gpu):
