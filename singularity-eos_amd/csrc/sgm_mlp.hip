@@ -9,7 +9,7 @@
 // from 56) -> 128 -> 128 -> 128 (GELU, tanh form) -> 16 outputs (6 slot
 // logits, masked softmax over the cell's materials; standardized ln T; 9
 // padding). BF16 weights and activations, FP32 accumulation on the matrix
-// cores (v_mfma_f32_16x16x16_bf16), FP32 master weights, Adam.
+// cores (gfx950's v_mfma_f32_16x16x32_bf16), FP32 master weights, Adam.
 //
 // Work split: a block of 4 waves owns 64 rows (cells); each wave computes 16
 // rows x all columns of a layer with 16x16 MFMA tiles; weights of the layer
@@ -35,6 +35,7 @@ constexpr int QW1 = 0, QW2 = QW1 + H * KIN, QW3 = QW2 + H * H, QW4 = QW3 + H * H
               NQ = QW4 + KOUT * H;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -63,15 +64,29 @@ __device__ inline float gelu_d(float z) {
 }
 
 // One wave: acc[j] (+)= A[16 rows x K] * Bt[16 cols of tile j x K]^T, bf16 in
-// LDS. A row-major with stride lda; Bt row-major ([N][K]) with stride ldb.
-// Lane l holds A[l%16][4(l/16)..+3] and Bt[l%16][4(l/16)..+3] of each 16x16
-// step; D[4(l/16)+i][l%16] comes back in acc[j][i].
+// LDS. A row-major with stride lda; Bt row-major ([N][K]) with stride ldb
+// (16-byte aligned rows). gfx950's 16x16x32 step: lane l holds
+// A[l%16][8(l/16)..+7] and Bt[l%16][8(l/16)..+7]; a K that leaves 16 over
+// (the 16-wide output layer's gradient product) ends with one 16x16x16 step,
+// lane l holding 4(l/16)..+3 of those 16. D[4(l/16)+i][l%16] comes back in
+// acc[j][i].
 template <int NT>
 __device__ inline void wave_gemm(const bf16 *A, int lda, const bf16 *Bt, int ldb, int K,
                                  f4 (&acc)[NT]) {
   const int l = threadIdx.x & 63;
-  const int r = l & 15, q = (l >> 4) * 4;
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  const int r = l & 15;
+  int k0 = 0;
+  for (; k0 + 32 <= K; k0 += 32) {
+    const int q = (l >> 4) * 8;
+    const b8 a = *reinterpret_cast<const b8 *>(A + r * lda + k0 + q);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const b8 b = *reinterpret_cast<const b8 *>(Bt + (j * 16 + r) * ldb + k0 + q);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+    }
+  }
+  if (k0 < K) {
+    const int q = (l >> 4) * 4;
     const s4 a = *reinterpret_cast<const s4 *>(A + r * lda + k0 + q);
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -157,9 +172,9 @@ __global__ void __launch_bounds__(256) fwd_kernel(const bf16 *__restrict__ wq,
                                                   const float *__restrict__ P,
                                                   const bf16 *__restrict__ Xall,
                                                   const int *__restrict__ idx, long B, FwdOut o) {
-  __shared__ bf16 X[ROWS * LDK];
-  __shared__ bf16 act0[ROWS * LDH], act1[ROWS * LDH];
-  __shared__ bf16 wl[H * LDH];
+  __shared__ __attribute__((aligned(16))) bf16 X[ROWS * LDK];
+  __shared__ __attribute__((aligned(16))) bf16 act0[ROWS * LDH], act1[ROWS * LDH];
+  __shared__ __attribute__((aligned(16))) bf16 wl[H * LDH];
   __shared__ float ylds[ROWS * KOUT];
   const long row0 = (long)blockIdx.x * ROWS;
   const int nrow = (int)((B - row0) < ROWS ? (B - row0) : ROWS);
@@ -228,10 +243,10 @@ struct BwdArgs {
 };
 
 __global__ void __launch_bounds__(256) bwd_kernel(BwdArgs a) {
-  __shared__ bf16 dZr[ROWS * LDH];   // [r][n]
-  __shared__ bf16 dZt[H * LDR];      // [n][r]
-  __shared__ bf16 At[H * LDR];       // [k][r]
-  __shared__ bf16 Wio[H * LDH];      // [k][n] : Bt for dA = dZ W
+  __shared__ __attribute__((aligned(16))) bf16 dZr[ROWS * LDH];   // [r][n]
+  __shared__ __attribute__((aligned(16))) bf16 dZt[H * LDR];      // [n][r]
+  __shared__ __attribute__((aligned(16))) bf16 At[H * LDR];       // [k][r]
+  __shared__ __attribute__((aligned(16))) bf16 Wio[H * LDH];      // [k][n] : Bt for dA = dZ W
   const int layer = a.layer;
   const int nout = layer == 4 ? KOUT : H, nin = layer == 1 ? KIN : H;
   const long row0 = (long)blockIdx.x * ROWS;
@@ -427,9 +442,9 @@ __global__ void __launch_bounds__(256) guess_kernel(PteArgs a, const bf16 *__res
                                                     const float *__restrict__ P,
                                                     const float *mu, const float *sg, float t_mu,
                                                     float t_sg, double *gv, double *gT) {
-  __shared__ bf16 X[ROWS * LDK];
-  __shared__ bf16 act0[ROWS * LDH], act1[ROWS * LDH];
-  __shared__ bf16 wl[H * LDH];
+  __shared__ __attribute__((aligned(16))) bf16 X[ROWS * LDK];
+  __shared__ __attribute__((aligned(16))) bf16 act0[ROWS * LDH], act1[ROWS * LDH];
+  __shared__ __attribute__((aligned(16))) bf16 wl[H * LDH];
   __shared__ float ylds[ROWS * KOUT];
   __shared__ int snm[ROWS];
   const long c0 = (long)blockIdx.x * ROWS;
