@@ -7,7 +7,8 @@
 //         per-cell inputs into one merged CSR, each vector request's inputs
 //   ker   offsets rebase (merged CSR), sort keys, radix sort, bin bounds,
 //         counter reset, one persistent bin kernel per material count 2..6
-//         and the list kernel for the rest (bin bounds read on the device);
+//         with straggler parking and the bin's tail kernels behind it, and
+//         the list kernel for the rest (bin bounds read on the device);
 //         one vector kernel per vector request
 //   D2H   results straight into the clients' arrays
 // Admission is first-come first-served: the queue head waits until its
@@ -32,6 +33,8 @@ using namespace sgm;
 namespace {
 
 constexpr size_t ALIGN = 256;
+// Newton iterations before a bin kernel parks a cell (as plans do)
+constexpr int SERVER_PARK_AFTER = 20;
 inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 
 // device-side description of one closure request inside a group
@@ -90,6 +93,24 @@ __global__ void ranges_kernel(const unsigned long long *__restrict__ keys, long 
   }
   range[12] = pos[5];
   range[13] = n - pos[5];
+}
+
+// The straggler tables of a launch group. The group's bins are not split by
+// material signature, so each bin is one run: every position maps to run 0
+// (run_of), the tail's wave slots walk the bin's records in park order
+// (wave_run 0, wave_first = slot x per_wave), and per bin `small` holds
+// {run_base 0, run_cap R, run_count 0, level-2 count 0, level-2 queue 0}.
+__global__ void park_tables_kernel(int *__restrict__ run_of, long ncell,
+                                   int *__restrict__ wave_run, int *__restrict__ wave_first,
+                                   int nw, int per_wave, int *__restrict__ small, int nsmall,
+                                   int R) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ncell) run_of[g] = 0;
+  if (g < nw) {
+    wave_run[g] = 0;
+    wave_first[g] = (int)g * per_wave;
+  }
+  if (g < nsmall) small[g] = (g % 5 == 1) ? R : 0;
 }
 
 struct Arena {
@@ -151,6 +172,11 @@ struct Layout {
   int npte = 0;
   size_t reqs, raw, off, mat, rho, vfrac, sie, temp, press, vt, st, conv, keys, keys2, ids,
       sorted, range, counters, sort_tmp, sort_tmp_bytes;
+  // straggler parking of the batched bins (2..6 materials): R records per bin
+  long park_R = 0;
+  int park_nw = 0, park_per_wave = 0;
+  size_t run_of = 0, wave_run = 0, wave_first = 0, park_small = 0;
+  size_t rec[7] = {}, cell_of[7] = {};
   std::vector<size_t> vec_in, vec_out;  // per vector request: rho, sie | P, T
   size_t total = 0;
 };
@@ -242,6 +268,23 @@ Layout make_layout(const sgm_server *sv, const std::vector<uint64_t> &ids) {
     L.counters = take(sizeof(int) * 8);
     L.sort_tmp_bytes = sort_tmp_bytes(L.ncell);
     L.sort_tmp = take(L.sort_tmp_bytes);
+    if (!sv->cfg.plain) {
+      // records for a sixteenth of the group's cells per bin (at least 1024):
+      // a cell still iterating after SERVER_PARK_AFTER Newton iterations moves
+      // to the bin's tail kernels while records last, and keeps its lane past
+      // that
+      L.park_R = std::min<long>(L.ncell, std::max<long>(1024, L.ncell / 16));
+      L.park_per_wave = 64 / TailGroup(sv->cfg.method, 2);
+      L.park_nw = (int)((L.park_R + L.park_per_wave - 1) / L.park_per_wave);
+      L.run_of = take(4 * L.ncell);
+      L.wave_run = take(4 * (size_t)L.park_nw);
+      L.wave_first = take(4 * (size_t)L.park_nw);
+      L.park_small = take(4 * 5 * 5);
+      for (int nm = 2; nm <= 6; ++nm) {
+        L.rec[nm] = take(8 * (size_t)ParkStride(sv->cfg.method, nm) * L.park_R);
+        L.cell_of[nm] = take(4 * (size_t)L.park_R);
+      }
+    }
   }
   for (uint64_t t : ids) {
     const Rec &r = sv->recs[t];
@@ -332,10 +375,36 @@ int launch_group(sgm_server *sv, Lane &ln, const Layout &L) {
       ok(hipGetLastError());
       ok(launch_zero_ints(counters, 8, s));
       const int grid = std::max(1, sv->ncu * 2);  // 8 waves per CU of 256-thread blocks
+      int *small = (int *)D(L.park_small);
+      const long nt = std::max<long>(std::max<long>(L.ncell, L.park_nw), 25);
+      park_tables_kernel<<<(int)((nt + 255) / 256), 256, 0, s>>>(
+          (int *)D(L.run_of), L.ncell, (int *)D(L.wave_run), (int *)D(L.wave_first), L.park_nw,
+          L.park_per_wave, small, 25, (int)L.park_R);
+      ok(hipGetLastError());
+      // each bin's persistent kernel parks its stragglers, then the bin's
+      // tail kernels finish them (as a plan does, on the lane's stream), so
+      // a few wandering cells no longer hold the bin kernel's whole grid
       for (int nm = 2; nm <= 6; ++nm) {
         const long *r = range + 2 * (nm - 1);
+        ParkArgs pk;
+        int *sm = small + 5 * (nm - 2);
+        pk.after = SERVER_PARK_AFTER;
+        pk.stride = ParkStride(sv->cfg.method, nm);
+        pk.rec = (double *)D(L.rec[nm]);
+        pk.cell_of = (int *)D(L.cell_of[nm]);
+        pk.run_of = (const int *)D(L.run_of);
+        pk.run_base = sm;
+        pk.run_cap = sm + 1;
+        pk.run_count = sm + 2;
+        pk.wave_run = (const int *)D(L.wave_run);
+        pk.wave_first = (const int *)D(L.wave_first);
+        pk.nwaves = L.park_nw;
+        pk.after2 = 4;
+        pk.l2_count = sm + 3;
+        pk.l2_next = sm + 4;
         ok(launch_pte_bin(sv->cfg.method, nm, a, sv->P, sorted, 0, counters + nm, 256, 64, grid,
-                          s, r));
+                          s, r, &pk));
+        ok(launch_pte_tail(sv->cfg.method, nm, a, sv->P, pk, s));
       }
       for (int side = 0; side < 2; ++side) {
         const long *r = range + (side ? 12 : 0);
