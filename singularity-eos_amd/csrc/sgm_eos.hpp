@@ -839,122 +839,149 @@ SGM_HD void base_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, 
 // FillEos for the (rho, sie) -> {T, P, cv, bmod} and (rho, T) -> {sie, P, cv, bmod}
 // input modes (eos_ideal.hpp:236-262, eos_stiff.hpp:203-229, eos_gruneisen.hpp:527-549,
 // eos_vinet.hpp:373-395, eos_spiner_rho_temp.hpp:1052-1101) plus (P, T) -> {rho, sie}.
-SGM_HD void base_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
-                     Real &bmod, unsigned long output, Lam &lam, int &err) {
+// FillEos per model (the bodies of each model's FillEos, eos_base.hpp and
+// the model headers), one function per kind so that a kernel built for one
+// kind holds that model's code alone
+SGM_HD void fill_gas(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
+                     Real &cv, Real &bmod, unsigned long output, Lam &, int &err) {
+  using namespace tq;
+  const bool st = e.kind == KIND_STIFF;
+  if (output & density && output & specific_internal_energy) {
+    if (output & pressure || output & temperature) {
+      err |= ERRB_FILLEOS;
+      return;
+    }
+    if (st)
+      stiff::rhoe_PT(e.m.p, press, temp, rho, sie);
+    else
+      ideal::rhoe_PT(e.m.p, press, temp, rho, sie);
+  }
+  if (output & pressure && output & specific_internal_energy) {
+    if (output & density || output & temperature) {
+      err |= ERRB_FILLEOS;
+      return;
+    }
+    sie = st ? stiff::e_rT(e.m.p, rho, temp) : ideal::e_rT(e.m.p, rho, temp);
+  }
+  if (output & temperature && output & specific_internal_energy) {
+    sie = st ? robust::ratio(press + (e.m.p[1] + 1.0) * e.m.p[2], e.m.p[1] * rho) + e.m.p[3]
+             : press / (e.m.p[1] * rho);
+  }
+  if (output & pressure) press = st ? stiff::P_re(e.m.p, rho, sie) : ideal::P_re(e.m.p, rho, sie);
+  if (output & temperature) temp = st ? stiff::T_re(e.m.p, rho, sie) : ideal::T_re(e.m.p, rho, sie);
+  if (output & bulk_modulus)
+    bmod = st ? stiff::bmod_re(e.m.p, rho, sie) : ideal::bmod_re(e.m.p, rho, sie);
+  if (output & specific_heat) cv = e.m.p[0];
+}
+SGM_HD void fill_grun(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
+                     Real &cv, Real &bmod, unsigned long output, Lam &lam, int &err) {
   using namespace tq;
   const unsigned long input = ~output;
+  if (temperature & input && pressure & input) {
+    grun::rhoe_PT(e.m.p, press, temp, rho, sie, err);
+  } else if (density & output || specific_internal_energy & output) {
+    err |= ERRB_FILLEOS;
+    return;
+  }
+  const Real r = smin(rho, e.m.p[10]);
+  if (output & pressure) press = grun::P_re(e.m.p, r, sie);
+  if (output & temperature) temp = grun::T_re(e.m.p, r, sie);
+  if (output & bulk_modulus) bmod = grun::bmod_re(e.m.p, r, sie);
+  if (output & specific_heat) cv = e.m.p[9];
+  return;
+}
+SGM_HD void fill_vinet(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
+                     Real &cv, Real &bmod, unsigned long output, Lam &lam, int &err) {
+  using namespace tq;
+  const unsigned long input = ~output;
+  if (density & output) {
+    err |= ERRB_FILLEOS;
+    return;
+  } else if (temperature & output && specific_internal_energy & output) {
+    err |= ERRB_FILLEOS;
+    return;
+  }
+  if (specific_internal_energy & input)
+    temp = vinet::T_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
+  const bool need_cs = (output & bulk_modulus) != 0;
+  vinet::EP o = need_cs ? vinet::F_DT<true>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp)
+                        : vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp);
+  if (output & specific_internal_energy) sie = o.e;
+  if (output & pressure) press = o.P;
+  if (output & specific_heat) cv = e.m.p[5];
+  if (output & bulk_modulus) {
+    const Real cs = sqrt(o.cs2);
+    bmod = cs * cs * rho;
+  }
+  return;
+}
+SGM_HD void fill_table(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press,
+                     Real &cv, Real &bmod, unsigned long output, Lam &lam, int &err) {
+  using namespace tq;
+  const unsigned long input = ~output;
+  const TableDev &t = e.tab;
+  Real lRho, lT;
+  int w;
+  if (output == 0) {
+    err |= ERRB_FILLEOS;
+    return;
+  }
+  if (output & density) {
+    if (input & pressure && input & temperature) {
+      lT = table::lT_(t, temp);
+      const table::InvRho iv = table::lRho_from_P_lT(t, press, lT, lam);
+      lam = iv.lam;
+      lRho = iv.lRho;
+      rho = table::rho_(t, lRho);
+    } else {
+      err |= ERRB_FILLEOS;
+      return;
+    }
+  } else {
+    lRho = table::lRho_(t, rho);
+  }
+  if (output & temperature) {
+    if (input & density && input & specific_internal_energy) {
+      const table::InvT iv = table::lT_from_lRho_sie(t, lRho, sie, lam);
+      lam = iv.lam;
+      lT = iv.lT;
+    } else {
+      err |= ERRB_FILLEOS;
+      return;
+    }
+    temp = table::T_(t, lT);
+  } else {
+    lT = table::lT_(t, temp);
+  }
+  w = table::loc_rho_T(t, lRho, lT);
+  if (output & specific_internal_energy) sie = table::sie_from(t, lRho, temp, lT, w);
+  if (output & pressure) press = table::P_from(t, rho, lRho, temp, lT, w);
+  if (output & specific_heat) cv = table::cv_from(t, lRho, lT, w);
+  if (output & bulk_modulus) bmod = table::bmod_from(t, rho, lRho, temp, lT, w);
+  lam.set(lRho, lT);
+}
+SGM_HD void base_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+                     Real &bmod, unsigned long output, Lam &lam, int &err) {
   switch (e.kind) {
   case KIND_IDEAL:
-  case KIND_STIFF: {
-    const bool st = e.kind == KIND_STIFF;
-    if (output & density && output & specific_internal_energy) {
-      if (output & pressure || output & temperature) {
-        err |= ERRB_FILLEOS;
-        return;
-      }
-      if (st)
-        stiff::rhoe_PT(e.m.p, press, temp, rho, sie);
-      else
-        ideal::rhoe_PT(e.m.p, press, temp, rho, sie);
-    }
-    if (output & pressure && output & specific_internal_energy) {
-      if (output & density || output & temperature) {
-        err |= ERRB_FILLEOS;
-        return;
-      }
-      sie = st ? stiff::e_rT(e.m.p, rho, temp) : ideal::e_rT(e.m.p, rho, temp);
-    }
-    if (output & temperature && output & specific_internal_energy) {
-      sie = st ? robust::ratio(press + (e.m.p[1] + 1.0) * e.m.p[2], e.m.p[1] * rho) + e.m.p[3]
-               : press / (e.m.p[1] * rho);
-    }
-    if (output & pressure) press = st ? stiff::P_re(e.m.p, rho, sie) : ideal::P_re(e.m.p, rho, sie);
-    if (output & temperature) temp = st ? stiff::T_re(e.m.p, rho, sie) : ideal::T_re(e.m.p, rho, sie);
-    if (output & bulk_modulus)
-      bmod = st ? stiff::bmod_re(e.m.p, rho, sie) : ideal::bmod_re(e.m.p, rho, sie);
-    if (output & specific_heat) cv = e.m.p[0];
-    return;
+  case KIND_STIFF: fill_gas(e, rho, temp, sie, press, cv, bmod, output, lam, err); return;
+  case KIND_GRUN: fill_grun(e, rho, temp, sie, press, cv, bmod, output, lam, err); return;
+  case KIND_VINET: fill_vinet(e, rho, temp, sie, press, cv, bmod, output, lam, err); return;
+  default: fill_table(e, rho, temp, sie, press, cv, bmod, output, lam, err); return;
   }
-  case KIND_GRUN: {
-    if (temperature & input && pressure & input) {
-      grun::rhoe_PT(e.m.p, press, temp, rho, sie, err);
-    } else if (density & output || specific_internal_energy & output) {
-      err |= ERRB_FILLEOS;
-      return;
-    }
-    const Real r = smin(rho, e.m.p[10]);
-    if (output & pressure) press = grun::P_re(e.m.p, r, sie);
-    if (output & temperature) temp = grun::T_re(e.m.p, r, sie);
-    if (output & bulk_modulus) bmod = grun::bmod_re(e.m.p, r, sie);
-    if (output & specific_heat) cv = e.m.p[9];
-    return;
-  }
-  case KIND_VINET: {
-    if (density & output) {
-      err |= ERRB_FILLEOS;
-      return;
-    } else if (temperature & output && specific_internal_energy & output) {
-      err |= ERRB_FILLEOS;
-      return;
-    }
-    if (specific_internal_energy & input)
-      temp = vinet::T_re(e.m.p, e.m.coef, e.nvinet_skip, rho, sie);
-    const bool need_cs = (output & bulk_modulus) != 0;
-    vinet::EP o = need_cs ? vinet::F_DT<true>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp)
-                          : vinet::F_DT<false>(e.m.p, e.m.coef, e.nvinet_skip, rho, temp);
-    if (output & specific_internal_energy) sie = o.e;
-    if (output & pressure) press = o.P;
-    if (output & specific_heat) cv = e.m.p[5];
-    if (output & bulk_modulus) {
-      const Real cs = sqrt(o.cs2);
-      bmod = cs * cs * rho;
-    }
-    return;
-  }
-  default: {
-    const TableDev &t = e.tab;
-    Real lRho, lT;
-    int w;
-    if (output == 0) {
-      err |= ERRB_FILLEOS;
-      return;
-    }
-    if (output & density) {
-      if (input & pressure && input & temperature) {
-        lT = table::lT_(t, temp);
-        const table::InvRho iv = table::lRho_from_P_lT(t, press, lT, lam);
-        lam = iv.lam;
-        lRho = iv.lRho;
-        rho = table::rho_(t, lRho);
-      } else {
-        err |= ERRB_FILLEOS;
-        return;
-      }
-    } else {
-      lRho = table::lRho_(t, rho);
-    }
-    if (output & temperature) {
-      if (input & density && input & specific_internal_energy) {
-        const table::InvT iv = table::lT_from_lRho_sie(t, lRho, sie, lam);
-        lam = iv.lam;
-        lT = iv.lT;
-      } else {
-        err |= ERRB_FILLEOS;
-        return;
-      }
-      temp = table::T_(t, lT);
-    } else {
-      lT = table::lT_(t, temp);
-    }
-    w = table::loc_rho_T(t, lRho, lT);
-    if (output & specific_internal_energy) sie = table::sie_from(t, lRho, temp, lT, w);
-    if (output & pressure) press = table::P_from(t, rho, lRho, temp, lT, w);
-    if (output & specific_heat) cv = table::cv_from(t, lRho, lT, w);
-    if (output & bulk_modulus) bmod = table::bmod_from(t, rho, lRho, temp, lT, w);
-    lam.set(lRho, lT);
-    return;
-  }
-  }
+}
+// base_fill for a material known to be of kind KIND
+template <int KIND>
+SGM_HD void base_fill_k(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+                        Real &bmod, unsigned long output, Lam &lam, int &err) {
+  if constexpr (KIND == KIND_IDEAL || KIND == KIND_STIFF)
+    fill_gas(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+  else if constexpr (KIND == KIND_GRUN)
+    fill_grun(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+  else if constexpr (KIND == KIND_VINET)
+    fill_vinet(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+  else
+    fill_table(e, rho, temp, sie, press, cv, bmod, output, lam, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1087,10 +1114,18 @@ SGM_HD void eos_rhoe_PT(const EosDev &e, Real P, Real T, Real &rho, Real &sie, L
 // shifted_eos.hpp:142-158): by the model's preferred input, (rho, T) or
 // (rho, sie); a modified IdealGas (whose preferred input is neither) is the
 // reference's EOS_ERROR, here ERRB_FILLEOS
-SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
-                     Real &bmod, unsigned long output, Lam &lam, int &err) {
+// eos_fill for a material of kind KIND (KIND < 0: any kind)
+template <int KIND>
+SGM_HD void eos_fill_k(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+                       Real &bmod, unsigned long output, Lam &lam, int &err) {
+  auto fill = [&](Real &r, Real &s) {
+    if constexpr (KIND < 0)
+      base_fill(e, r, temp, s, press, cv, bmod, output, lam, err);
+    else
+      base_fill_k<KIND>(e, r, temp, s, press, cv, bmod, output, lam, err);
+  };
   if (!e.mods) {
-    base_fill(e, rho, temp, sie, press, cv, bmod, output, lam, err);
+    fill(rho, sie);
     return;
   }
   const unsigned long pref = base_preferred_input(e);
@@ -1101,9 +1136,13 @@ SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &pr
   }
   Real srho = mod_rho(e, rho);
   Real senergy = rhoT ? sie : mod_sie(e, sie);
-  base_fill(e, srho, temp, senergy, press, cv, bmod, output, lam, err);
+  fill(srho, senergy);
   if (!(e.mods & MOD_SCALED)) rho = srho;  // an unscaled model fills the caller's rho
   if (rhoT) sie = mod_energy_out(e, senergy);
+}
+SGM_HD void eos_fill(const EosDev &e, Real &rho, Real &temp, Real &sie, Real &press, Real &cv,
+                     Real &bmod, unsigned long output, Lam &lam, int &err) {
+  eos_fill_k<-1>(e, rho, temp, sie, press, cv, bmod, output, lam, err);
 }
 
 

@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256)
       lam.lT = lp[1];
     }
     int err = 0;
-    eos_fill(ek, r, t, s, p, c, b, output, lam, err);
+    eos_fill_k<KIND>(ek, r, t, s, p, c, b, output, lam, err);
     // an output set the model cannot fill (the reference's EOS_ERROR) writes
     // nothing: the point's arrays keep their values
     // (a modified material's FillEos may also change an input in place, as
