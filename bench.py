@@ -165,6 +165,9 @@ def main():
     ap.add_argument("--learned-train-cells", type=int, default=1 << 22)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run the serving loop's traces alone and print their summary "
+                         "(for profiling the server); not the bench line")
     ap.add_argument("--copy-cal", action="store_true",
                     help="end with one elementwise kernel that reads 1 GiB and writes 1 GiB "
                          "(torch.mul; the scale for FETCH_SIZE / WRITE_SIZE counters)")
@@ -196,6 +199,10 @@ def main():
     mat = torch.tensor(a["mat_ids"], device=dev)
     conv = torch.zeros(n, dtype=torch.int32, device=dev)
     niter = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve_bench(torch, ctx, a, params, args.serve_requests)}))
+        return
 
     # the plan (sorting the cells into material-count bins, straggler
     # records, graph capture on first execute) is built once per cell
