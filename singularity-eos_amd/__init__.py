@@ -440,7 +440,7 @@ class Server:
     must stay alive until the request is done (the Server keeps a reference)."""
 
     def __init__(self, ctx, budget_bytes=8 << 30, nlanes=4, max_group_requests=64,
-                 max_group_cells=1 << 21, plain=False, method=0, params=None):
+                 max_group_cells=1 << 22, plain=False, method=0, params=None):
         self.h = vp()
         self.ctx = ctx
         cfg = ServerConfig(budget_bytes, nlanes, max_group_requests, max_group_cells,

@@ -92,7 +92,7 @@ def make_trace(nreq, rate, seed, pool_cells, nmat=6, pte_frac=0.3, nmin=1e3, nma
 
 
 def run(torch, ctx, pools, trace, plain=False, budget=8 << 30, nlanes=4, params=None,
-        ring_bytes=3 << 30, max_group_cells=1 << 21):
+        ring_bytes=3 << 30, max_group_cells=1 << 22):
     """Replays a trace against a fresh server. Returns per-request latency
     (s), total cells, makespan and server stats."""
     srv = Server(ctx, budget_bytes=budget, nlanes=nlanes, plain=plain, params=params,
