@@ -197,7 +197,8 @@ hipError_t reset_counters(sgm_plan *p, hipStream_t s) {
   hipError_t e = launch_zero_ints(p->d_counters, (int)p->bins.size(), s);
   if (e == hipSuccess && p->park_after > 0 && p->nruns > 0)
     e = launch_zero_ints(p->d_run_count, p->nruns, s);
-  if (e == hipSuccess && p->park_after > 0 && p->d_l2_counts)
+  // cleared with parking off too: sgm_plan_tail_counts then reports zeros
+  if (e == hipSuccess && p->d_l2_counts)
     e = launch_zero_ints(p->d_l2_counts, 2 * (int)p->bins.size(), s);
   return e;
 }

@@ -322,3 +322,44 @@ def test_straggler_parking_keeps_results(sgm, workloads, pool, torch_gpu, method
         assert np.array_equal(out["err"], g["err"])
         assert np.array_equal(out["resid"], g["resid"])
     plan.close()
+
+
+def test_plan_tail_counts(sgm, workloads, pool, torch_gpu):
+    """sgm_plan_tail_counts: per bin, the cells the last execution's tail
+    resumed in its second level -- the same on a repeated execution, zero
+    with parking off and for list bins, and at most the bin's cells whose
+    Newton loop ran 4 iterations past the straggler threshold (each moved
+    unless a fixed point let it finish in the first level)."""
+    torch = torch_gpu
+    ctx, _, _ = pool
+    a = workloads.uneven(ctx, 1 << 18)
+    d = dev(torch, a)
+    nm_cell = np.diff(a["offsets"])
+    plan = sgm.Plan(ctx, a["ncell"], a["nnz"], d["offsets"], d["mat_ids"], 0)
+    params = sgm.mix_params(tight=True)
+    init = {k: d[k].clone() for k in FIELDS}
+    counts, niters = [], []
+    for thr in (20, 20, 0):
+        plan.set_straggler_threshold(thr)
+        for k in FIELDS:
+            d[k].copy_(init[k])
+        rc = plan.execute(d["rho"], d["vfrac"], d["sie"], d["temp"], d["press"], d["vfrac_tot"],
+                          d["sie_tot"], params=params, conv=d["conv"], niter=d["niter"])
+        assert rc == 0
+        torch.cuda.synchronize()
+        counts.append(plan.tail_counts())
+        niters.append(d["niter"].cpu().numpy())
+    info = plan.describe()
+    plan.close()
+    assert counts[0] == counts[1]
+    assert not any(counts[2])
+    assert np.array_equal(niters[0], niters[2])  # the schedule changes nothing
+    moved = 0
+    for b, c in zip(info, counts[0]):
+        if not 2 <= b["nmat"] <= 6:
+            assert c == 0
+            continue
+        late = int((niters[0][nm_cell == b["nmat"]] >= 24).sum())
+        assert 0 <= c <= late, (b["nmat"], c, late)
+        moved += c
+    assert moved > 0
