@@ -188,8 +188,8 @@ int sgm_vec_rhoe_from_PT(sgm_ctx *ctx, int matindex, const double *P, const doub
  * [offsets[c], offsets[c+1]) of the per-(cell,material) arrays; mat_ids holds
  * context material indices; 1..64 materials per cell (1..8 one lane per cell
  * with the state in registers, 9..64 one wave per cell with the Newton system
- * in LDS; 9+ material cells of sgm_pte_rhoU are flagged SGM_ERRB_INPUT, its
- * formulation having no one-wave form). Replaces constructing
+ * in LDS; sgm_pte_rhoU, whose 2 nm equations must fit that workspace, up to
+ * 32 and flags wider cells SGM_ERRB_INPUT). Replaces constructing
  * PTESolverRhoT / PTESolverPT(nmat, eos, vfrac_tot, sie_tot, rho, vfrac, sie,
  * temp, press, lambda, scratch, Tnorm, params) and calling PTESolver
  * (mixed_cell_models.hpp:922-935, 1245-1253, 2188-2291) once per cell.

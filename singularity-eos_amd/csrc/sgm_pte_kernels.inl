@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) pte_plain_kernel(PteArgs a, MixParamsDev 
     case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
     default:
       // cells of 9..64 materials are the wave kernel's
-      if (!(nm > MAX_MATS_PER_CELL && wave::HasWaveForm(METHOD))) mark_bad(a, c);
+      if (!(nm > MAX_MATS_PER_CELL && nm <= wave::WaveMaxNM(METHOD))) mark_bad(a, c);
       break;
     }
   }
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(256) pte_list_kernel(PteArgs a, MixParamsDev P
     case 8: solve_cell_call<8, METHOD>(lds, a, P, c, off); break;
     default:
       // cells of 9..64 materials are the wave kernel's
-      if (!(nm > MAX_MATS_PER_CELL && wave::HasWaveForm(METHOD))) mark_bad(a, c);
+      if (!(nm > MAX_MATS_PER_CELL && nm <= wave::WaveMaxNM(METHOD))) mark_bad(a, c);
       break;
     }
   }
@@ -713,8 +713,8 @@ __device__ __forceinline__ void wave_solve_cell(const PteArgs &a, const MixParam
 
 // Cells of 9..64 materials, one wave (the whole 64-lane workgroup) per cell:
 // each wave reads 64 list entries at a time and solves the wide cells among
-// them one after another; the others it leaves to the plain and list
-// kernels. A wave that finds none never stages the registry. LDS: the
+// them one after another (RhoU's up to 32 materials; it flags the wider
+// ones as malformed); the others it leaves to the plain and list kernels. A wave that finds none never stages the registry. LDS: the
 // registry, then the wave's Newton system. cells == nullptr: cells 0..n-1.
 template <int METHOD>
 __global__ void __launch_bounds__(64) pte_wave_kernel(PteArgs a, MixParamsDev P,
@@ -744,8 +744,11 @@ __global__ void __launch_bounds__(64) pte_wave_kernel(PteArgs a, MixParamsDev P,
     while (wide) {
       const int src = __builtin_ctzll(wide);
       wide &= wide - 1;
-      wave_solve_cell<METHOD>(a, P, wave::at(c, src), wave::at(off, src), wave::at(nm, src),
-                              work, lane);
+      const int k = wave::at(nm, src);
+      if (k <= wave::WaveMaxNM(METHOD))
+        wave_solve_cell<METHOD>(a, P, wave::at(c, src), wave::at(off, src), k, work, lane);
+      else if (lane == 0)
+        mark_bad(a, wave::at(c, src));  // past the formulation's one-wave size
     }
   }
 }
@@ -753,17 +756,13 @@ __global__ void __launch_bounds__(64) pte_wave_kernel(PteArgs a, MixParamsDev P,
 template <int METHOD>
 hipError_t wave_launch(const PteArgs &a, const MixParamsDev &P, const int *cells, long n,
                        hipStream_t s, const long *range) {
-  if constexpr (!wave::HasWaveForm(METHOD)) {
-    return hipSuccess;  // the plain and list kernels flag such cells
-  } else {
-    if (n <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.nreg * sizeof(EosDev) + sizeof(Real) * wave::WORK_WORDS;
-    const long blocks_needed = (n + 63) / 64;
-    const long cap = (long)device_cu_count() * 8;
-    const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
-    pte_wave_kernel<METHOD><<<g, 64, lds, s>>>(a, P, cells, n, range);
-    return hipGetLastError();
-  }
+  if (n <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.nreg * sizeof(EosDev) + sizeof(Real) * wave::WORK_WORDS;
+  const long blocks_needed = (n + 63) / 64;
+  const long cap = (long)device_cu_count() * 8;
+  const int g = (int)(blocks_needed < cap ? blocks_needed : cap);
+  pte_wave_kernel<METHOD><<<g, 64, lds, s>>>(a, P, cells, n, range);
+  return hipGetLastError();
 }
 } // namespace kern
 } // namespace sgm

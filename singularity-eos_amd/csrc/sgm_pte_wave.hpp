@@ -1000,15 +1000,254 @@ struct FixedP : Base<MODS> {
   __device__ Real Pout() const { return press * uscale; }
 };
 
-// formulation by method id (0 RhoT, 1 PT, 2 FixedT, 3 FixedP); RhoU has no
-// one-wave form
+// ---------------------------------------------------------------------------
+// PTESolverRhoU (:1924-2183, with GetIdealPTE / TryIdealPTE :672-763):
+// unknowns the volume fractions (columns 0..n-1) and scaled energy densities
+// (n..2n-1) of every material; rows: volume and energy closure, neighbouring
+// pressure and temperature differences. 2n equations, so at most 32
+// materials fit the wave's workspace.
+// ---------------------------------------------------------------------------
+template <bool MODS>
+struct RhoU : Base<MODS> {
+  using Ev = sgm::Ev<MODS>;
+  using B = Base<MODS>;
+  using B::act;
+  using B::dx;
+  using B::jac;
+  using B::lam;
+  using B::lane;
+  using B::n;
+  using B::P;
+  using B::press;
+  using B::res;
+  using B::rho;
+  using B::rhobar;
+  using B::sie;
+  using B::Tnorm;
+  using B::u;
+  using B::uscale;
+  using B::utotal_scale;
+  using B::vfrac;
+  static constexpr bool kStoresTemp = true;
+  static __device__ int Equations(int nm) { return 2 * nm; }
+  Real temp = 0, vtemp = 0, utemp = 0, dtdv = 0, dtde = 0;
+
+  __device__ void SetCellInputs(Real vfrac_tot, Real sie_tot, Real Tnorm_in) {
+    B::vfrac_total = vfrac_tot;
+    B::sie_total = sie_tot;
+    Tnorm = Tnorm_in;
+  }
+  // GetPressureFromPreferred (:583-597)
+  __device__ Real PressureFromPreferred(Real r, Real T, Real s) {
+    const EosDev &e = B::E();
+    if (Ev::preferred_input(e) & tq::specific_internal_energy) return Ev::P_re(e, r, s, lam);
+    return Ev::P_rT(e, r, T, lam);
+  }
+  // :2014-2026
+  __device__ void Residual() {
+    const Real vs = B::NSum(vfrac), es = B::NSum(u);
+    const int nx = lane + 1 < 64 ? lane + 1 : lane;
+    const Real pnext = __shfl(press, nx, 64), tnext = __shfl(temp, nx, 64);
+    wsync();
+    if (lane == 0) {
+      res[0] = B::vfrac_total - vs;
+      res[1] = utotal_scale - es;
+    }
+    if (lane < n - 1) {
+      res[2 + lane] = pnext - press;
+      res[n + 1 + lane] = tnext - temp;
+    }
+    wsync();
+  }
+  // :672-687
+  __device__ void GetIdealPTE(Real &Pideal, Real &Tideal) const {
+    const Real a = vfrac * robust::ratio(press, temp);
+    const Real rb = rho * vfrac * robust::ratio(sie, temp);
+    Real Asum = 0.0, rhoBsum = 0.0;
+    for (int k = 0; k < n; ++k) {
+      Asum += at(a, k);
+      rhoBsum += at(rb, k);
+    }
+    Asum *= uscale / Tnorm;
+    rhoBsum /= Tnorm;
+    Tideal = uni(uscale / rhoBsum / Tnorm);
+    Pideal = uni(robust::ratio(Tnorm * Tideal * Asum, uscale * B::vfrac_total));
+  }
+  __device__ Real SumSquaresRes() const {
+    Real r = 0.0;
+    for (int i = 0; i < 2 * n; ++i) r += res[i] * res[i];
+    return uni(r);
+  }
+  // :691-763. A material whose scaled volume fraction falls below its
+  // minimum-pressure density stops the reference's loop and restores the
+  // fractions it changed: none changes then.
+  __device__ void TryIdealPTE() {
+    Real Pideal, Tideal;
+    GetIdealPTE(Pideal, Tideal);
+    const Real etemp = u, ptemp = press, vt = vfrac, rtemp = rho;
+    Real *rs = B::w.x;  // the residual before the ideal-gas guess
+    for (int i = lane; i < 2 * n; i += 64) rs[i] = res[i];
+    const Real res_norm_old = SumSquaresRes();
+    const Real alpha = robust::ratio(Pideal, Tideal);
+    const Real vnew = vfrac * robust::ratio(press, (temp * alpha));
+    const bool bad = Tnorm * Tideal < 0 ||
+                     robust::ratio(rhobar, vnew) < Ev::rho_pmin(B::E(), Tnorm * Tideal);
+    if (any_lane(act && bad)) return;
+    vfrac = vnew;
+    rho = robust::ratio(rhobar, vfrac);
+    Real P_T;
+    const Real sie_m = Ev::e_rT_with_P(B::E(), rho, Tnorm * Tideal, lam, P_T);
+    u = rhobar * robust::ratio(sie_m, uscale);
+    press = robust::ratio(B::PressureFromPreferredP(rho, sie_m, P_T), uscale);
+    Residual();
+    const Real res_norm_new = SumSquaresRes();
+    if (res_norm_new > res_norm_old) {
+      vfrac = vt;
+      rho = rtemp;
+      u = etemp;
+      press = ptemp;
+      for (int i = lane; i < 2 * n; i += 64) res[i] = rs[i];
+      wsync();
+    } else {
+      temp = Tideal;
+      sie = uscale * robust::ratio(u, rhobar);
+    }
+  }
+  // :2005-2011 (InitBase sets every scaled temperature to 1, :641)
+  __device__ Real Init(Real Tin) {
+    B::InitBase(Tin);
+    temp = 1.0;
+    Residual();
+    TryIdealPTE();
+    return B::ResidualNorm();
+  }
+  __device__ void Fixup() {}
+  // :2029-2055
+  __device__ void CheckPTE(bool &conv, bool &ok) const {
+    const Real vp = vfrac * press, rt = rhobar * temp;
+    Real mean_p = at(vp, 0), mean_t = at(rt, 0);
+    Real error_p = 0.0, error_t = 0.0;
+    for (int k = 1; k < n; ++k) {
+      mean_p += at(vp, k);
+      mean_t += at(rt, k);
+      error_p += res[k + 1] * res[k + 1];
+      error_t += res[k + n] * res[k + n];
+    }
+    mean_t = robust::ratio(mean_t, B::rho_total);
+    error_p = sqrt(error_p);
+    error_t = sqrt(error_t);
+    const bool cp = (error_p < P.pte_rel_tolerance_p * fabs(mean_p) ||
+                     error_p < P.pte_abs_tolerance_p);
+    const bool ct = (error_t < P.pte_rel_tolerance_t * mean_t || error_t < P.pte_abs_tolerance_t);
+    const bool ap = (error_p < P.pte_rel_tolerance_p_sufficient * fabs(mean_p) ||
+                     error_p < P.pte_abs_tolerance_p_sufficient);
+    const bool at_ = (error_t < P.pte_rel_tolerance_t_sufficient * mean_t ||
+                      error_t < P.pte_abs_tolerance_t_sufficient);
+    conv = uni(cp && ct);
+    ok = uni(ap && at_);
+  }
+  // :2058-2121: this lane's derivatives, then its columns m and n + m
+  __device__ void Jacobian() {
+    const EosDev &e = B::E();
+    const Real dv = (vfrac < 0.5 ? 1.0 : -1.0) * vfrac * P.derivative_eps;
+    const Real vf_pert = vfrac + dv;
+    const Real rho_pert = robust::ratio(rhobar, vf_pert);
+    Real t_pert = robust::ratio(Ev::T_re(e, rho_pert, sie, lam), Tnorm);
+    Real p_pert = robust::ratio(PressureFromPreferred(rho_pert, Tnorm * t_pert, sie), uscale);
+    const Real dpdv = smin(robust::ratio(p_pert - press, dv), -fabs(P.pte_max_dpdv));
+    dtdv = robust::ratio(t_pert - temp, dv);
+    const Real de = fabs(u) * P.derivative_eps;
+    const Real e_pert = robust::ratio(u + de, rhobar);
+    t_pert = robust::ratio(Ev::T_re(e, rho, uscale * e_pert, lam), Tnorm);
+    p_pert = robust::ratio(PressureFromPreferred(rho, Tnorm * t_pert, uscale * e_pert), uscale);
+    const Real dpde = robust::ratio(p_pert - press, de);
+    dtde = robust::ratio(t_pert - temp, de);
+    if (fabs(dtde) < P.min_dtde) dtde = P.derivative_eps;
+    const int N = 2 * n, m = lane;
+    B::ZeroJac();
+    if (act) {
+      jac[m] = 1.0;
+      jac[2 * n + n + m] = 1.0;
+      if (m == 0) {
+        jac[2 * N] = dpdv;
+        jac[(n + 1) * N] = dtdv;
+        jac[2 * N + n] = dpde;
+        jac[(n + 1) * N + n] = dtde;
+      } else if (m == n - 1) {
+        jac[n * N + n - 1] = -dpdv;
+        jac[(2 * n - 1) * N + n - 1] = -dtdv;
+        jac[n * N + 2 * n - 1] = -dpde;
+        jac[(2 * n - 1) * N + 2 * n - 1] = -dtde;
+      } else {
+        jac[(1 + m) * N + m] = -dpdv;
+        jac[(2 + m) * N + m] = dpdv;
+        jac[(n + m) * N + m] = -dtdv;
+        jac[(n + m + 1) * N + m] = dtdv;
+        jac[(1 + m) * N + n + m] = -dpde;
+        jac[(2 + m) * N + n + m] = dpde;
+        jac[(n + m) * N + n + m] = -dtde;
+        jac[(n + m + 1) * N + n + m] = dtde;
+      }
+    }
+    wsync();
+  }
+  // :2124-2158: the limits chain through the materials in order, each
+  // material's minimum-pressure density at the temperature the chain has
+  // reached (every lane evaluates its material there; lane k's is used)
+  __device__ Real ScaleDx() {
+    Real scale = 1.0;
+    const Real rp0 = Ev::rho_pmin(B::E(), Tnorm * temp);
+    for (int k = 0; k < n; ++k) {
+      const Real v = at(vfrac, k), tk = at(temp, k), d = dx[k], dE = dx[k + n];
+      if (scale * d < -0.1 * v) scale = -0.1 * robust::ratio(v, d);
+      const Real dt = (at(dtdv, k) * d + at(dtde, k) * dE);
+      if (scale * dt < -0.1 * tk) scale = -0.1 * robust::ratio(tk, dt);
+      const Real tt = uni(tk + scale * dt);
+      const Real rho_min = smax(at(rp0, k), at(Ev::rho_pmin(B::E(), Tnorm * tt), k));
+      const Real alpha_max = robust::ratio(at(rhobar, k), rho_min);
+      if (!(alpha_max < v)) {
+        if (scale * d > 0.5 * (alpha_max - v)) scale = 0.5 * robust::ratio(alpha_max - v, d);
+      }
+      scale = uni(scale);
+    }
+    wsync();
+    for (int i = lane; i < 2 * n; i += 64) dx[i] *= scale;
+    wsync();
+    return scale;
+  }
+  // :2162-2181
+  __device__ Real TestUpdate(Real scale, bool cache_state) {
+    if (cache_state) {
+      vtemp = vfrac;
+      utemp = u;
+    }
+    const int m = act ? lane : 0;
+    vfrac = vtemp + scale * dx[m];
+    rho = robust::ratio(rhobar, vfrac);
+    u = utemp + scale * dx[n + m];
+    sie = uscale * robust::ratio(u, rhobar);
+    temp = robust::ratio(Ev::T_re(B::E(), rho, sie, lam), Tnorm);
+    press = robust::ratio(PressureFromPreferred(rho, Tnorm * temp, sie), uscale);
+    Residual();
+    return B::ResidualNorm();
+  }
+  __device__ Real Tout() const { return temp * Tnorm; }
+  __device__ Real Pout() const { return press * uscale; }
+};
+
+// formulation by method id (0 RhoT, 1 PT, 2 FixedT, 3 FixedP, 4 RhoU)
 template <int METHOD, bool MODS = true>
 using SysOf = typename std::conditional<
     METHOD == 0, RhoT<MODS>,
     typename std::conditional<
         METHOD == 1, PT<MODS>,
-        typename std::conditional<METHOD == 2, FixedT<MODS>, FixedP<MODS>>::type>::type>::type;
-constexpr bool HasWaveForm(int method) { return method >= 0 && method <= 3; }
+        typename std::conditional<
+            METHOD == 2, FixedT<MODS>,
+            typename std::conditional<METHOD == 3, FixedP<MODS>, RhoU<MODS>>::type>::type>::
+            type>::type;
+// the largest material count of the one-wave form by method: RhoU's 2 nm
+// equations fit the workspace up to 32 materials
+constexpr int WaveMaxNM(int method) { return method == 4 ? 32 : MAX_MATS; }
 
 } // namespace wave
 } // namespace sgm

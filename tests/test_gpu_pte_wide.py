@@ -1,8 +1,8 @@
-"""GPU parity of the one-wave-per-cell closure -- cells of 9..64 materials --
-against the oracle, through the C ABI: the plain per-call entry points in the
-four formulations that have the one-wave form (RhoT, PT, FixedT, FixedP),
-batched plans whose cells mix the register-resident counts with wide ones,
-and get_sg_eos with 14 material slots in every input mode.
+"""GPU parity of the one-wave-per-cell closure -- cells of 9..64 materials,
+9..32 for RhoU -- against the oracle, through the C ABI: the plain per-call
+entry points in all five formulations, batched plans whose cells mix the
+register-resident counts with wide ones, and get_sg_eos with 14 material
+slots in every input mode.
 
 Bar: bit for bit -- states, convergence flags, iteration counts and error
 bits -- as for the narrower cells: the one-wave solvers form every value in
@@ -109,11 +109,16 @@ def wide_cells(o, ref, counts, seed, method=0):
 EDGE_COUNTS = [9, 10, 12, 14, 16, 20, 24, 31, 32, 33, 47, 63, 64]
 
 
-@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def edge_counts(method):
+    """EDGE_COUNTS within the formulation's one-wave size (RhoU: 32)."""
+    return [c for c in EDGE_COUNTS if method != 4 or c <= 32]
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
 def test_wide_plain_bit_exact(sgm, oracle, wide_pool, torch_gpu, method):
     ctx, o, ref, _ = wide_pool
     rng = np.random.default_rng(0x5EED0000 + 90 + method)
-    counts = EDGE_COUNTS + list(rng.integers(9, 15, 96))
+    counts = edge_counts(method) + list(rng.integers(9, 15, 96))
     a = wide_cells(o, ref, counts, 0x5EED0000 + 91 + method, method)
     for tight in (False, True):
         params = sgm.mix_params(tight=tight)
@@ -128,25 +133,28 @@ def test_wide_plain_bit_exact(sgm, oracle, wide_pool, torch_gpu, method):
         print(f"[wide] method {method} tight {tight} converged {g['conv'].mean():.3f}", flush=True)
 
 
-def test_wide_rhou_flagged(sgm, oracle, wide_pool, torch_gpu):
-    """RhoU has no one-wave form: its cells of 9+ materials come back flagged
-    as malformed (status bit 32, not converged), the narrower ones solved."""
+def test_wide_rhou_past_32_flagged(sgm, oracle, wide_pool, torch_gpu):
+    """RhoU's 2 nm equations fit the one-wave workspace up to 32 materials:
+    its wider cells come back flagged as malformed (status bit 32, not
+    converged) in the plain path and in a plan, the others solved."""
     ctx, o, ref, _ = wide_pool
-    a = wide_cells(o, ref, [3, 9, 2, 12], 0x5EED0000 + 97)
-    g = run_plain(torch_gpu, ctx, a, 4, sgm.mix_params())
-    assert list(g["err"][[1, 3]] & 32) == [32, 32]
-    assert list(g["conv"][[1, 3]]) == [0, 0]
-    assert (g["err"][[0, 2]] & 32).sum() == 0
+    a = wide_cells(o, ref, [3, 33, 2, 40, 32, 64, 9], 0x5EED0000 + 97)
+    wide = [1, 3, 5]
+    for g in (run_plain(torch_gpu, ctx, a, 4, sgm.mix_params()),
+              run_batched(torch_gpu, sgm, ctx, a, 4, sgm.mix_params())[0]):
+        assert list(g["err"][wide] & 32) == [32, 32, 32]
+        assert list(g["conv"][wide]) == [0, 0, 0]
+        assert (g["err"][[0, 2, 4, 6]] & 32).sum() == 0
 
 
-@pytest.mark.parametrize("method", [0, 2])
+@pytest.mark.parametrize("method", [0, 2, 4])
 def test_wide_cells_in_plans(sgm, oracle, wide_pool, torch_gpu, method):
     """A plan over cells of 1..64 materials: the wide ones form a list bin of
     their own (one wave per cell), the rest the usual bins; results equal the
     plain path and the oracle."""
     ctx, o, ref, _ = wide_pool
     rng = np.random.default_rng(0x5EED0000 + 93 + method)
-    counts = list(rng.integers(1, 9, 400)) + EDGE_COUNTS + list(rng.integers(9, 17, 40))
+    counts = list(rng.integers(1, 9, 400)) + edge_counts(method) + list(rng.integers(9, 17, 40))
     rng.shuffle(counts)
     a = wide_cells(o, ref, counts, 0x5EED0000 + 94 + method, method)
     params = sgm.mix_params(tight=True)
