@@ -15,20 +15,11 @@ namespace sgm {
 // log2 T (until the bracket is narrower than 0.01) and a linear
 // interpolation of log2 T by u inside the bracket. eidx[m] is material m's
 // registry index; lam may be NULL (no table caches).
-SGM_HD Real approx_temperature_from_rho_mat_u(int nmat, const unsigned char *eidx, Real u_tot,
-                                              const Real *rho, const Real *vfrac, Real Tguess,
-                                              Lam *lam) {
+// ufunc(T): the cell's energy density sum at temperature T
+template <class U>
+SGM_HD Real approx_temperature_core(Real u_tot, Real Tguess, U &&ufunc) {
   constexpr Real minimum_temperature = 1.e-9;
   constexpr Real maximum_temperature = 1.e9;
-  auto ufunc = [&](Real T) {
-    Real usum = 0.0;
-    for (int m = 0; m < nmat; ++m) {
-      Lam none;
-      Lam &l = lam ? lam[m] : none;
-      usum += rho[m] * vfrac[m] * eos_e_rT(registry_entry(eidx[m]), rho[m], T, l);
-    }
-    return usum;
-  };
   Real ulo = ufunc(minimum_temperature);
   if (u_tot < ulo) return minimum_temperature;
   Real uhi = ufunc(maximum_temperature);
@@ -58,6 +49,20 @@ SGM_HD Real approx_temperature_from_rho_mat_u(int nmat, const unsigned char *eid
   }
   const Real alpha = robust::ratio((u_tot - ulo), (uhi - ulo));
   return nqt::pow2((1.0 - alpha) * lTlo + alpha * lThi);
+}
+SGM_HD Real approx_temperature_from_rho_mat_u(int nmat, const unsigned char *eidx, Real u_tot,
+                                              const Real *rho, const Real *vfrac, Real Tguess,
+                                              Lam *lam) {
+  auto ufunc = [&](Real T) {
+    Real usum = 0.0;
+    for (int m = 0; m < nmat; ++m) {
+      Lam none;
+      Lam &l = lam ? lam[m] : none;
+      usum += rho[m] * vfrac[m] * eos_e_rT(registry_entry(eidx[m]), rho[m], T, l);
+    }
+    return usum;
+  };
+  return approx_temperature_core(u_tot, Tguess, ufunc);
 }
 
 // the per-cell kernel of sgm_approx_temperature (sgm_approx_temp.hip)

@@ -1,4 +1,5 @@
-// ApproxTemperatureFromRhoMatU over a CSR list of cells: one lane per cell,
+// ApproxTemperatureFromRhoMatU over a CSR list of cells (1..64 materials):
+// one lane per cell,
 // the material registry staged once per block into LDS (every EOS parameter
 // read of the bisection is an LDS access). Per cell it reads the entries'
 // densities and volume fractions and the cell's energy density, and writes
@@ -27,6 +28,36 @@ __global__ void __launch_bounds__(256)
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += nthr) {
     const long o0 = offsets[c], o1 = offsets[c + 1];
     const long nm = o1 - o0;
+    if (o0 >= 0 && o1 >= o0 && o1 <= nnz && nm > MAX_MATS_PER_CELL && nm <= MAX_MATS_WAVE) {
+      // a wider cell: its entries are read where they lie, the lambda caches
+      // read and written back around each evaluation (the same values as the
+      // register copies of a narrower cell)
+      bool ok = true;
+      for (long k = o0; k < o1; ++k) ok = ok && mat[k] < nreg;
+      if (!ok) {
+        T_out[c] = __builtin_nan("");
+        continue;
+      }
+      auto ufunc = [&](Real T) {
+        Real usum = 0.0;
+        for (long k = o0; k < o1; ++k) {
+          Lam l;
+          if (lambda) {
+            l.present = true;
+            l.lRho = lambda[3 * k + 0];
+            l.lT = lambda[3 * k + 1];
+          }
+          usum += rho[k] * vfrac[k] * eos_e_rT(registry_entry(mat[k]), rho[k], T, l);
+          if (lambda) {
+            lambda[3 * k + 0] = l.lRho;
+            lambda[3 * k + 1] = l.lT;
+          }
+        }
+        return usum;
+      };
+      T_out[c] = approx_temperature_core(u_tot[c], Tguess ? Tguess[c] : 0.0, ufunc);
+      continue;
+    }
     bool ok = o0 >= 0 && o1 >= o0 && o1 <= nnz && nm >= 1 && nm <= MAX_MATS_PER_CELL;
     unsigned char ids[MAX_MATS_PER_CELL];
     Real r[MAX_MATS_PER_CELL], v[MAX_MATS_PER_CELL];

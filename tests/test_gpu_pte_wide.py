@@ -232,3 +232,44 @@ def test_driver_wide_bit_exact(sgm, oracle, wide_pool, torch_gpu, mode):
         assert same.all(), (mode, k, int((~same).sum()))
     assert np.array_equal(conv.cpu().numpy(), conv_o)
     print(f"[wide driver] mode {mode} converged {conv_o.mean():.3f}", flush=True)
+
+
+def test_wide_approx_temperature(sgm, oracle, wide_pool, torch_gpu):
+    """ApproxTemperatureFromRhoMatU (sgm_approx_temperature) on cells of 9..64
+    materials against the oracle's, bit for bit, without and with a guess,
+    and with lambda caches (which change no value)."""
+    import ctypes as C
+    torch = torch_gpu
+    ctx, o, ref, _ = wide_pool
+    counts = EDGE_COUNTS + [3, 8, 11]
+    a = wide_cells(o, ref, counts, 0x5EED0000 + 99)
+    n, nnz, off = a["ncell"], a["nnz"], a["offsets"]
+    rb = a["rho"] * a["vfrac"]
+    cell = np.repeat(np.arange(n), np.diff(off))
+    u = np.bincount(cell, weights=rb, minlength=n) * a["sie_tot"]
+    d = {k: torch.tensor(a[k], device="cuda", dtype=torch.float64) for k in ("rho", "vfrac")}
+    offs = torch.tensor(off, device="cuda", dtype=torch.int32)
+    mats = torch.tensor(a["mat_ids"], device="cuda", dtype=torch.uint8)
+    ud = torch.tensor(u, device="cuda", dtype=torch.float64)
+
+    def oracle_T(guess):
+        out = np.zeros(n)
+        for c in range(n):
+            s = slice(off[c], off[c + 1])
+            ids = a["mat_ids"][s]
+            hs = (C.c_void_p * len(ids))(*[o[i].h for i in ids])
+            out[c] = oracle.lib().sgo_approx_temperature(
+                len(ids), hs, float(u[c]), oracle.p(np.ascontiguousarray(a["rho"][s])),
+                oracle.p(np.ascontiguousarray(a["vfrac"][s])), float(guess[c]))
+        return out
+
+    r0 = oracle_T(np.zeros(n))
+    for guess, lam in ((None, None), (r0 * 1.3, None), (None, True)):
+        T = torch.zeros(n, dtype=torch.float64, device="cuda")
+        gd = torch.tensor(guess, device="cuda") if guess is not None else None
+        ld = torch.zeros(3 * nnz, dtype=torch.float64, device="cuda") if lam else None
+        assert ctx.approx_temperature(n, nnz, offs, mats, d["rho"], d["vfrac"], ud, T,
+                                      Tguess=gd, lam=ld) == 0
+        torch.cuda.synchronize()
+        want = r0 if guess is None else oracle_T(guess)
+        assert np.array_equal(T.cpu().numpy(), want)
