@@ -502,6 +502,16 @@ int sgm_plan_execute_guided(sgm_plan *plan, double *rho, double *vfrac, double *
                             const double *sie_tot, const double *guess_vfrac,
                             const double *guess_T, const sgm_mix_params *params,
                             sgm_pte_status status, int *used_guess, sgm_stream_t stream);
+/* No reference counterpart: sgm_mlp_guess and sgm_plan_execute_guided as one
+ * launch -- the plan's graph runs the network's guess kernel (into the
+ * caller's guess_vfrac [nnz] / guess_T [ncell]) and then the guided bins.
+ * Results equal sgm_mlp_guess followed by sgm_plan_execute_guided. RhoT and
+ * PT plans. */
+int sgm_plan_execute_learned(sgm_plan *plan, sgm_mlp *mlp, double *rho, double *vfrac,
+                             double *sie, double *temp, double *press, const double *vfrac_tot,
+                             const double *sie_tot, double *guess_vfrac, double *guess_T,
+                             const sgm_mix_params *params, sgm_pte_status status,
+                             int *used_guess, sgm_stream_t stream);
 
 /* ---- defaults and misc -------------------------------------------------- */
 void sgm_mix_params_default(sgm_mix_params *p);

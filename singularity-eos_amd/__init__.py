@@ -156,6 +156,8 @@ _SIGS = {
     "sgm_mlp_guess": (ci, [vp, vp, cl, cl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "sgm_plan_execute_guided": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                      C.POINTER(MixParams), PteStatus, vp, vp]),
+    "sgm_plan_execute_learned": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                      C.POINTER(MixParams), PteStatus, vp, vp]),
     "sgm_server_create": (ci, [vp, C.POINTER(ServerConfig), C.POINTER(MixParams),
                                C.POINTER(vp)]),
     "sgm_server_submit": (ci, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
@@ -376,6 +378,17 @@ class Plan:
                                              ptr(guess_vfrac), ptr(guess_T),
                                              C.byref(params) if params is not None else None,
                                              st, ptr(used), stream)
+
+    def execute_learned(self, mlp, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot,
+                        guess_vfrac, guess_T, params=None, conv=None, niter=None, resid=None,
+                        err=None, used=None, stream=0):
+        """The network's guess and the guided solve in one graph launch."""
+        st = PteStatus(ptr(conv), ptr(niter), ptr(resid), ptr(err))
+        return lib().sgm_plan_execute_learned(self.h, mlp, ptr(rho), ptr(vfrac), ptr(sie),
+                                              ptr(temp), ptr(press), ptr(vfrac_tot),
+                                              ptr(sie_tot), ptr(guess_vfrac), ptr(guess_T),
+                                              C.byref(params) if params is not None else None,
+                                              st, ptr(used), stream)
 
     def execute_bin(self, b, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, Tnorm=None,
                     lam=None, params=None, conv=None, niter=None, resid=None, err=None,

@@ -27,7 +27,7 @@ struct Bin {
 };
 
 struct GraphKey {
-  const void *ptrs[16];
+  const void *ptrs[17];
   MixParamsDev P;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -654,19 +654,23 @@ int sgm_plan_autotune(sgm_plan *p, const double *rho, const double *vfrac, const
 } // extern "C"
 
 namespace {
+// The plan's graph for these arguments (captured when they change) launched
+// on s. With an MLP, the graph first runs the network's guess kernel into
+// guess_vfrac / guess_T, so inference and the guided solve are one launch.
 int execute_common(sgm_plan *p, double *rho, double *vfrac, double *sie, double *temp,
                    double *press, const double *vfrac_tot, const double *sie_tot,
                    const double *Tnorm, double *lambda, const sgm_mix_params *params,
                    sgm_pte_status status, const double *guess_vfrac, const double *guess_T,
-                   int *used_guess, hipStream_t s) {
+                   int *used_guess, hipStream_t s, sgm_mlp *mlp = nullptr) {
   const MixParamsDev P = params_dev(params);
   GraphKey k;
   std::memset(&k, 0, sizeof(k));
-  const void *ptrs[16] = {rho,         vfrac,        sie,          temp,
+  const void *ptrs[17] = {rho,         vfrac,        sie,          temp,
                           status.residual, press,    vfrac_tot,    sie_tot,
                           Tnorm,       lambda,       status.converged, status.niter,
-                          status.err,  guess_vfrac,  guess_T,      used_guess};
-  for (int i = 0; i < 16; ++i) k.ptrs[i] = ptrs[i];
+                          status.err,  guess_vfrac,  guess_T,      used_guess,
+                          mlp};
+  for (int i = 0; i < 17; ++i) k.ptrs[i] = ptrs[i];
   k.P = P;
   if (!p->have_key || !(p->key == k) || !p->exec) {
     if (p->exec) hipGraphExecDestroy(p->exec);
@@ -680,7 +684,13 @@ int execute_common(sgm_plan *p, double *rho, double *vfrac, double *sie, double 
     a.used_guess = used_guess;
     if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return SGM_E_HIP;
-    const hipError_t le = launch_bins(p, a, P, p->cap_stream, true);
+    hipError_t le = hipSuccess;
+    if (mlp &&
+        sgm_mlp_guess(mlp, p->ctx, p->ncell, p->nnz, p->d_offsets, p->d_mat, rho, vfrac, temp,
+                      vfrac_tot, sie_tot, const_cast<double *>(guess_vfrac),
+                      const_cast<double *>(guess_T), (sgm_stream_t)p->cap_stream) != SGM_OK)
+      le = hipErrorLaunchFailure;
+    if (le == hipSuccess) le = launch_bins(p, a, P, p->cap_stream, true);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(p->cap_stream, &g);
     if (le != hipSuccess || ce != hipSuccess || !g) {
@@ -724,6 +734,21 @@ int sgm_plan_execute_guided(sgm_plan *p, double *rho, double *vfrac, double *sie
   if (p->method != PTE_RHOT && p->method != PTE_PT) return SGM_E_ARG;
   return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, nullptr, nullptr,
                         params, status, guess_vfrac, guess_T, used_guess, (hipStream_t)stream);
+}
+
+int sgm_plan_execute_learned(sgm_plan *p, sgm_mlp *mlp, double *rho, double *vfrac, double *sie,
+                             double *temp, double *press, const double *vfrac_tot,
+                             const double *sie_tot, double *guess_vfrac, double *guess_T,
+                             const sgm_mix_params *params, sgm_pte_status status,
+                             int *used_guess, sgm_stream_t stream) {
+  if (!p || !mlp || !rho || !vfrac || !sie || !temp || !press || !vfrac_tot || !sie_tot ||
+      !guess_vfrac || !guess_T)
+    return SGM_E_ARG;
+  if (p->mod_mask != ctx_mod_mask(p->ctx)) return SGM_E_ARG;  // modifiers changed
+  if (p->method != PTE_RHOT && p->method != PTE_PT) return SGM_E_ARG;
+  return execute_common(p, rho, vfrac, sie, temp, press, vfrac_tot, sie_tot, nullptr, nullptr,
+                        params, status, guess_vfrac, guess_T, used_guess, (hipStream_t)stream,
+                        mlp);
 }
 
 int sgm_plan_set_shape(sgm_plan *p, int bin, int block, int cells_per_wave, int grid) {

@@ -123,5 +123,25 @@ def test_training_reduces_newton_iterations(sgm, workloads, torch_gpu, L):
     T1 = r["temp"][off[:-1]]
     differ = both & (np.abs(T1 - T0) > 1e-6 * T0)
     assert differ.mean() < 1e-3, differ.sum()
+    # inference and the guided solve as one graph launch (twice: the second
+    # replays the captured graph) == the guess, then the guided execute
+    plan = sgm.Plan(ctx, n, nnz, d["offsets"], d["mat_ids"], 0)
+    for _ in range(2):
+        o2 = {k: d[k].clone() for k in ("rho", "vfrac", "sie", "temp", "press")}
+        gv2 = torch.zeros_like(gv)
+        gT2 = torch.zeros_like(gT)
+        cv2 = torch.zeros(n, dtype=torch.int32, device="cuda")
+        it2 = torch.zeros(n, dtype=torch.int32, device="cuda")
+        us2 = torch.zeros(n, dtype=torch.int32, device="cuda")
+        assert plan.execute_learned(mlp.h, o2["rho"], o2["vfrac"], o2["sie"], o2["temp"],
+                                    o2["press"], d["vfrac_tot"], d["sie_tot"], gv2, gT2,
+                                    params=params, conv=cv2, niter=it2, used=us2) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(gv2, gv) and torch.equal(gT2, gT)
+        for k in o2:
+            assert torch.equal(o2[k], out[k]), k
+        assert np.array_equal(it2.cpu().numpy(), it1)
+        assert np.array_equal(us2.cpu().numpy().astype(bool), used)
+    plan.close()
     mlp.close()
     ctx.close()
