@@ -8,9 +8,10 @@
 // pivoted QR over the lanes, each element again in the serial order.
 //
 // Restates singularity-eos/closure/mixed_cell_models.hpp: solve_Ax_b_wscr
-// :176-278, PTESolverBase :329-786, PTESolverRhoT :873-1193, PTESolverPT
-// :1198-1446, PTESolverFixedT :1451-1669, PTESolverFixedP :1674-1921, with
-// the driver loop of PTESolver :2188-2291 (pte_step).
+// :176-278, PTESolverBase :329-786 (GetIdealPTE / TryIdealPTE :672-763),
+// PTESolverRhoT :873-1193, PTESolverPT :1198-1446, PTESolverFixedT
+// :1451-1669, PTESolverFixedP :1674-1921, PTESolverRhoU :1924-2183, with the
+// driver loop of PTESolver :2188-2291 (pte_step).
 //
 // Lanes at or past the material count mirror material 0 so that their EOS
 // calls stay on valid inputs; nothing they compute is read.
