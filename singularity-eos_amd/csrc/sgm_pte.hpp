@@ -235,6 +235,180 @@ SGM_HD bool solve_small(Real (&a)[N * N], Real (&b)[N]) {
   return ok;
 }
 
+// lane k's x, k the same on every lane (two v_readlane)
+__device__ __forceinline__ Real lane_value(Real x, int k) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, k);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), k);
+  return __builtin_bit_cast(Real, ((unsigned long long)hi << 32) | lo);
+}
+
+// solve_small<N> by a whole wave whose lanes hold identical copies of a and
+// b: lane j < N keeps column j of the scaled matrix in registers and applies
+// the reflections and norm downdates to it; pivoting swaps entries of a
+// wave-uniform position -> lane map instead of moving columns; the pivot
+// search, each Householder vector, the reflection of b and the back
+// substitution read the columns they need with v_readlane and run alike on
+// every lane. Every value is formed by the operations of solve_small in
+// their order, so b (overwritten with x on every lane) and the result are
+// bit-identical to it. Needs every lane of the wave active.
+template <int N>
+__device__ bool solve_small_wave(const Real (&a)[N * N], Real (&b)[N], int lane) {
+  static_assert(N >= 3, "the wave form covers the QR sizes");
+  const int j = lane < N ? lane : N - 1;
+  Real rmax[N];
+  bool zero = false;
+#pragma unroll
+  for (int row = 0; row < N; ++row) {
+    Real maxabs = 0;
+#pragma unroll
+    for (int c = 0; c < N; ++c) maxabs = smax(fabs(a[row * N + c]), maxabs);
+    rmax[row] = maxabs;
+    zero = zero || maxabs == 0;
+  }
+  if (zero) return false;
+  Real col[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    Real v = opaque(a[i * N]);
+#pragma unroll
+    for (int c = 1; c < N; ++c) v = sel(j == c, a[i * N + c], v);
+    col[i] = v / rmax[i];
+    b[i] /= rmax[i];
+  }
+  Real cu, cd;
+  {
+    Real s = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += col[i] * col[i];
+    cd = sqrt(s);
+    cu = cd;
+  }
+  int L[N];  // position -> lane (= the column's original index)
+#pragma unroll
+  for (int p = 0; p < N; ++p) L[p] = p;
+  bool done = false;  // this lane's column is final (an R column)
+  const Real eps = 2.220446049250313e-16;
+  const Real norm_downdate_threshold = sqrt(eps);
+  int nzp = N;
+  Real th = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int big = k;
+    Real bign = lane_value(cu, L[k]);
+#pragma unroll
+    for (int p = k + 1; p < N; ++p) {
+      const Real c = lane_value(cu, L[p]);
+      if (c > bign) {
+        bign = c;
+        big = p;
+      }
+    }
+    big = __builtin_amdgcn_readfirstlane(big);
+    const Real big_sq = bign * bign;
+    if (k == 0) th = big_sq * eps * eps / N;
+    if (nzp == N && big_sq < th * (N - k)) nzp = k;
+#pragma unroll
+    for (int p = k + 1; p < N; ++p) {
+      if (big == p) {
+        const int t = L[k];
+        L[k] = L[p];
+        L[p] = t;
+      }
+    }
+    const int lk = L[k];
+    Real v[N];
+    Real tailsq = 0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      v[i] = lane_value(col[i], lk);
+      tailsq += v[i] * v[i];
+    }
+    const Real c0 = lane_value(col[k], lk);
+    Real tau, beta;
+    if (tailsq <= 2.2250738585072014e-308) {
+      tau = 0;
+      beta = c0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) v[i] = 0;
+    } else {
+      beta = sqrt(c0 * c0 + tailsq);
+      if (c0 >= 0) beta = -beta;
+      const Real den = c0 - beta;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) v[i] = v[i] / den;
+      tau = (beta - c0) / beta;
+    }
+    const bool mine = lane == lk;
+    const bool refl = lane < N && !done && !mine;
+    if (mine) {
+      col[k] = beta;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) col[i] = v[i];
+    }
+    if (refl) {
+      Real s = 0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) s += v[i] * col[i];
+      s += col[k];
+      col[k] -= tau * s;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) col[i] -= (tau * v[i]) * s;
+    }
+    {
+      Real s = 0;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) s += v[i] * b[i];
+      s += b[k];
+      b[k] -= tau * s;
+#pragma unroll
+      for (int i = k + 1; i < N; ++i) b[i] -= (tau * v[i]) * s;
+    }
+    if (refl && cu != 0) {
+      Real t = fabs(col[k]) / cu;
+      t = (1 + t) * (1 - t);
+      if (t < 0) t = 0;
+      const Real r = cu / cd;
+      const Real t2 = t * r * r;
+      if (t2 <= norm_downdate_threshold) {
+        Real s = 0;
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) s += col[i] * col[i];
+        cd = sqrt(s);
+        cu = cd;
+      } else {
+        cu *= sqrt(t);
+      }
+    }
+    done = done || mine;
+  }
+  Real z[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) z[i] = b[i];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    if (i < nzp) {
+      z[i] = z[i] / lane_value(col[i], L[i]);
+#pragma unroll
+      for (int r = 0; r < i; ++r) z[r] -= z[i] * lane_value(col[r], L[i]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < N; ++c) b[c] = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const Real zi = (i < nzp) ? z[i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+      if (L[i] == c) b[c] = zi;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (!isfinite_ref(b[i])) ok = false;
+  return ok;
+}
+
 // The closure kernels stage the material registry into LDS (dynamic shared
 // memory) once per block; cells name their materials by registry index, so
 // every EOS parameter read is an LDS access.
@@ -477,6 +651,16 @@ struct PteBase {
 #pragma unroll
     for (int i = 0; i < NEQ; ++i) dx[i] = res[i];
     return solve_small<NEQ>(jac, dx);
+  }
+  // Solve() by a whole wave holding identical copies of the cell
+  __device__ __forceinline__ bool SolveWave(int lane) {
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) dx[i] = res[i];
+    if constexpr (NEQ < 3) {
+      return solve_small<NEQ>(jac, dx);
+    } else {
+      return solve_small_wave<NEQ>(jac, dx, lane);
+    }
   }
 };
 
@@ -1960,7 +2144,7 @@ __device__ __forceinline__ bool pte_iterate_wide(S &s, DriverState &d, CellStatu
   s.CheckPTE(d.converged, d.close_enough);
   if (d.converged) return true;
   s.JacobianWide(lane);
-  if (!s.Solve()) {
+  if (!s.SolveWave(lane)) {
     d.converged = false;
     s.err |= ERRB_LINSOLVE;
     return true;

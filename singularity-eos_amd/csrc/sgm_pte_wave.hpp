@@ -35,12 +35,7 @@ __device__ __forceinline__ Real uni(Real x) {
 }
 __device__ __forceinline__ bool uni(bool x) { return __builtin_amdgcn_readfirstlane((int)x) != 0; }
 // lane k's value (k wave-uniform)
-__device__ __forceinline__ Real at(Real x, int k) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
-  return __builtin_bit_cast(Real, ((unsigned long long)hi << 32) | lo);
-}
+__device__ __forceinline__ Real at(Real x, int k) { return lane_value(x, k); }
 __device__ __forceinline__ int at(int x, int k) { return __builtin_amdgcn_readlane(x, k); }
 __device__ __forceinline__ bool any_lane(bool x) { return __ballot(x) != 0; }
 __device__ __forceinline__ bool all_lanes(bool x) { return __ballot(!x) == 0; }
