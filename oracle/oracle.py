@@ -94,6 +94,7 @@ _SIGS = {
     "sgo_regula_falsi_atan": (ci, [cd] * 8 + [vp, vp]),
     "sgo_findroot_cubic": (ci, [cd] * 6 + [vp]),
     "sgo_solve": (ci, [ci, vp, vp]),
+    "sgo_qr_colpiv": (None, [ci, vp, vp, vp, vp]),
     "sgo_sum_neumaier": (cd, [vp, ci, ci, ci]),
     "sgo_enforce_mass_volumes": (ci, [ci, cd, vp, vp]),
     "sgo_enforce_energies": (ci, [ci, cd, cd, vp, vp, vp]),

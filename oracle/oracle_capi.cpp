@@ -205,6 +205,11 @@ int sgo_solve(int n, double *a, double *b) {
   double scr[16];
   return mix::solve_Ax_b_wscr(n, a, b, scr) ? 1 : 0;
 }
+// the pivoted QR alone (no row scaling), n >= 1: a (row-major) is left
+// holding R in its upper triangle, x the solution, perm the pivot order
+void sgo_qr_colpiv(int n, double *a, double *b, double *x, int *perm) {
+  mix::qr_colpiv_solve(n, a, b, x, perm);
+}
 double sgo_sum_neumaier(const double *d, int n, int offset, int iskip) {
   return mix::sum_neumaier(d, n, offset, iskip);
 }

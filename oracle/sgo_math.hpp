@@ -451,7 +451,9 @@ inline Real sum_neumaier(const Real *data, int n, int offset = 0, int iskip = -1
 // threshold; Householder sign convention beta = -sign(c0)*||x||; pivots whose
 // squared norm falls below (eps*max_norm)^2/n*(n-k) are treated as zero and
 // their unknowns set to 0). Matrix is row-major n x n.
-inline void qr_colpiv_solve(const int n, Real *a, Real *b, Real *x) {
+// perm_out (optional) receives the pivot order: the original column at each
+// position of R.
+inline void qr_colpiv_solve(const int n, Real *a, Real *b, Real *x, int *perm_out = nullptr) {
   constexpr int NMAX = 128;  // neq <= 2 * 64 materials
   int perm[NMAX];
   Real cn_upd[NMAX], cn_dir[NMAX];
@@ -549,6 +551,8 @@ inline void qr_colpiv_solve(const int n, Real *a, Real *b, Real *x) {
   }
   for (int i = 0; i < n; ++i) x[i] = 0;
   for (int i = 0; i < rank; ++i) x[perm[i]] = z[i];
+  if (perm_out)
+    for (int i = 0; i < n; ++i) perm_out[i] = perm[i];
 }
 
 // solve_Ax_b_wscr — mixed_cell_models.hpp:176-278. a is row-major n x n,
