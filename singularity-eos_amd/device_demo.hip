@@ -5,7 +5,8 @@
 // per-material arrays, ApproxTemperatureFromRhoMatU for the temperature
 // guess, then PTESolverRhoT or PTESolverPT and PTESolver. Built into
 // libsgm_device_demo.so, which the tests call to check that the device-side
-// interface gives the library's results bit for bit.
+// interface gives the library's results bit for bit (and that the library's
+// two forms of its dense linear solve agree bit for bit).
 #define SGM_DEVICE_MAX_MATS 6
 #include "../include/sgm_device.hpp"
 
@@ -56,9 +57,66 @@ __global__ void __launch_bounds__(256)
   if (max_line) max_line[c] = (int)st.max_line_niter;
 }
 
+// The library's two forms of the dense solve on one system per wave: every
+// lane runs solve_small<N> (the register-resident bins' form) and the wave
+// runs solve_small_wave<N> (the whole-wave tails' form); lane 0 stores both
+// solutions, and the kernel counts the lanes whose wave-form result is not
+// lane 0's bit for bit.
+template <int N>
+__global__ void __launch_bounds__(64)
+    demo_solve_kernel(const double *A, const double *b, double *x_serial, double *x_wave,
+                      int *ok, int *lanes_differ) {
+  const long sys = blockIdx.x;  // the grid holds one block per system
+  const int lane = threadIdx.x;
+  Real a[N * N], bs[N], bw[N];
+#pragma unroll
+  for (int k = 0; k < N * N; ++k) a[k] = A[sys * N * N + k];
+#pragma unroll
+  for (int i = 0; i < N; ++i) bs[i] = bw[i] = b[sys * N + i];
+  const bool okw = sgm::solve_small_wave<N>(a, bw, lane);
+  const bool oks = sgm::solve_small<N>(a, bs);
+  bool differ = false;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    differ = differ || __builtin_bit_cast(unsigned long long, bw[i]) !=
+                           __builtin_bit_cast(unsigned long long, sgm::lane_value(bw[i], 0));
+  const unsigned long long mask = __ballot(differ);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      x_serial[sys * N + i] = bs[i];
+      x_wave[sys * N + i] = bw[i];
+    }
+    ok[sys] = (oks ? 1 : 0) | (okw ? 2 : 0);
+    lanes_differ[sys] = __popcll(mask);
+  }
+}
+
 } // namespace
 
 extern "C" {
+
+// Both dense solves (above) on `count` systems of n = 3..8 equations, A
+// row-major n x n per system, b n per system, every array device memory.
+// ok[s]: bit 0 the serial form succeeded, bit 1 the wave form did.
+int sgm_demo_solve_forms(int n, long count, const double *A, const double *b,
+                         double *x_serial, double *x_wave, int *ok, int *lanes_differ,
+                         sgm_stream_t stream) {
+  if (n < 3 || n > 8 || count < 0 || count > 2147483647L) return SGM_E_ARG;
+  if (count == 0) return SGM_OK;
+  if (!A || !b || !x_serial || !x_wave || !ok || !lanes_differ) return SGM_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)count);
+  switch (n) {
+  case 3: demo_solve_kernel<3><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  case 4: demo_solve_kernel<4><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  case 5: demo_solve_kernel<5><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  case 6: demo_solve_kernel<6><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  case 7: demo_solve_kernel<7><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  default: demo_solve_kernel<8><<<grid, 64, 0, s>>>(A, b, x_serial, x_wave, ok, lanes_differ); break;
+  }
+  return hipGetLastError() == hipSuccess ? SGM_OK : SGM_E_HIP;
+}
 
 // One closure per CSR cell (2..6 materials) through the device interface.
 // method 0 PTESolverRhoT, 1 PTESolverPT; approx_tnorm: the temperature guess
